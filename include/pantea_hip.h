@@ -1,0 +1,276 @@
+/*
+ * pantea_hip.h -- C ABI of libpantea_hip.so, the gfx950 (MI355X) kernels of
+ * the Behler-Parrinello NNP hot path.
+ *
+ * Conventions
+ *  - Every pointer argument marked (device) is a device pointer (for example
+ *    torch.Tensor.data_ptr() of a CUDA/HIP tensor); (host) pointers are read
+ *    before the call returns.
+ *  - Work is enqueued asynchronously on `stream` (a hipStream_t; pass
+ *    torch.cuda.current_stream().cuda_stream).  Only pt_nbr_build
+ *    synchronises `stream` (once, to read the pair count back).
+ *  - Return value: PT_OK (0) or a negative PT_ERR_*; pt_last_error() gives a
+ *    thread-local message.  Buffers are owned by the caller; scratch comes in
+ *    through `ws`/`ws_bytes`, sized by the *_workspace_bytes() queries.
+ *  - Atom order: the neighbour build returns `perm` (internal -> input
+ *    index) and all other entry points work on that internal order, in which
+ *    atoms are grouped by type (ascending) and, inside a type, by cell.
+ *
+ * The reference (pantea 0.11.0) is pure Python on JAX; it has no FFI.  Each
+ * entry point names the reference function it replaces (path:line in the
+ * reference tree).
+ */
+#ifndef PANTEA_HIP_H
+#define PANTEA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *pt_stream_t; /* hipStream_t */
+
+enum {
+    PT_OK = 0,
+    PT_ERR_ARG = -1,
+    PT_ERR_CAPACITY = -2,
+    PT_ERR_HIP = -3,
+    PT_ERR_UNSUPPORTED = -4
+};
+
+/* Arithmetic of the descriptor kernels. */
+enum { PT_F64 = 0, PT_F32 = 1 };
+
+/* Arithmetic of the MLP kernels (matrix cores). */
+enum {
+    PT_MLP_F64 = 0,    /* v_mfma_f64_16x16x4_f64 */
+    PT_MLP_BF16 = 1,   /* v_mfma_f32_16x16x32_bf16, one product */
+    PT_MLP_BF16X3 = 2  /* split bf16 (hi*hi + hi*lo + lo*hi), FP32 accumulate */
+};
+
+/* Cutoff function codes (n2p2 numbering, potentials/nnp/settings.py:43-51). */
+enum { PT_CUT_HARD = 0, PT_CUT_COS = 1, PT_CUT_TANHU = 2, PT_CUT_TANH = 3,
+       PT_CUT_EXP = 4, PT_CUT_POLY1 = 5, PT_CUT_POLY2 = 6 };
+
+/* MLP activation codes (models/nn/activation.py:48-60). */
+enum { PT_ACT_IDENTITY = 0, PT_ACT_TANH = 1, PT_ACT_LOGISTIC = 2, PT_ACT_SOFTPLUS = 3,
+       PT_ACT_RELU = 4, PT_ACT_GAUSSIAN = 5, PT_ACT_COS = 6, PT_ACT_EXP = 7,
+       PT_ACT_HARMONIC = 8 };
+
+/* Scaler kinds (descriptors/scaler.py:206-246). */
+enum { PT_SCALE_CENTER = 0, PT_SCALE_SCALE = 1, PT_SCALE_SCALE_CENTER = 2,
+       PT_SCALE_SCALE_CENTER_SIGMA = 3 };
+
+const char *pt_last_error(void);
+int pt_version(void);
+
+/* ------------------------------------------------------------------------
+ * Symmetry-function table of one ACSF descriptor (descriptors/acsf/acsf.py:
+ * 43-45), compiled on the host into groups that share their neighbour loop.
+ *  - radial group: functions of one neighbour type and one cutoff;
+ *    G1 is G2 with eta = 0, r_shift = 0 (radial.py:38-61).
+ *  - angular group: functions of one unordered neighbour-type pair, one
+ *    cutoff and one kind (G3/G9); each function is a cell (eta e, angular
+ *    term a) of an n_eta x n_ang grid; col[e*4+a] = output column or -1.
+ * The table lives in device memory (one upload per descriptor).
+ * ---------------------------------------------------------------------- */
+#define PT_RG_MAX 8
+#define PT_AG_ETA 4
+#define PT_AG_ANG 4
+#define PT_MAX_GROUPS 24
+
+typedef struct {
+    int elem;   /* neighbour atom type, 1-based */
+    int cut;    /* PT_CUT_* */
+    int n;      /* functions in the group (<= PT_RG_MAX) */
+    int pad_;
+    double rc;
+    double eta[PT_RG_MAX];
+    double rs[PT_RG_MAX];
+    int col[PT_RG_MAX];
+} pt_radial_group;
+
+typedef struct {
+    int ej, ek;   /* neighbour atom types, 1-based, ej <= ek */
+    int cut;      /* PT_CUT_* */
+    int kind;     /* 3 = G3, 9 = G9 (angular.py:50-107) */
+    int n_eta, n_ang;
+    double rc;
+    double half;  /* 0.5 when ej == ek (acsf.py:285-290), else 1 */
+    double eta[PT_AG_ETA];
+    double zeta[PT_AG_ANG];
+    double lam[PT_AG_ANG];
+    double pre[PT_AG_ANG];    /* 2^(1 - zeta) */
+    int izeta[PT_AG_ANG];     /* zeta as a small integer (0..16) or -1 */
+    int col[PT_AG_ETA * PT_AG_ANG];
+} pt_angular_group;
+
+typedef struct {
+    int n_sf;
+    int n_rg, n_ag;
+    int pad_;
+    double rc_max;
+    double tanh_pre; /* ((e + 1/e) / (e - 1/e))^3, cutoff.py:82 */
+    pt_radial_group rg[PT_MAX_GROUPS];
+    pt_angular_group ag[PT_MAX_GROUPS];
+} pt_sf_table;
+
+/* ------------------------------------------------------------------------
+ * (a) Neighbour list.  Replaces the dense cutoff masks of
+ * atoms/neighbor.py:47-64,102-107 (mask (r <= rc) & (r > 0) over the
+ * minimum-image distances of atoms/distance.py:63-78).
+ * Cell binning, stable LSD radix sort of (type, cell) keys, then a CSR list
+ * whose rows hold internal neighbour indices in ascending order.
+ *  pos      (device) n rows of pos_stride (>= 3) FP64, x y z first, input
+ *           order (pos_stride 4 lets a previous pos4 be the input);
+ *           types (device) n, 1-based
+ *  box      (host) 3 box lengths, or NULL for a non-periodic structure
+ *  perm     (device, out) n: internal -> input index
+ *  pos4     (device, out) n x 4 FP64: x, y, z, type (internal order)
+ *  row_off  (device, out) n + 1
+ *  nbr      (device, out) cap; rev (device, out, optional) cap: index of the
+ *           reverse edge j->i for every edge i->j
+ *  n_pairs, max_row (host, out)
+ * Returns PT_ERR_CAPACITY (with *n_pairs set) when cap is too small.
+ * ---------------------------------------------------------------------- */
+size_t pt_nbr_workspace_bytes(int n, int n_types);
+int pt_nbr_build(int n, const double *pos, int pos_stride, const int *types, int n_types,
+                 const double *box, double rc,
+                 int *perm, double *pos4, long long *row_off, int *nbr, int *rev,
+                 long long cap, long long *n_pairs, int *max_row,
+                 void *ws, size_t ws_bytes, pt_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * (b) ACSF forward: descriptor rows of the centres
+ *     centers[c] (or center_begin + c when centers == NULL), internal order.
+ * Replaces _calculate_acsf_descriptor (descriptors/acsf/acsf.py:163-213).
+ *  G (device, out): n_centers x ldg, FP64 (PT_F64) or FP32 (PT_F32).
+ *  max_row: the longest neighbour row (from pt_nbr_build).
+ * ---------------------------------------------------------------------- */
+int pt_acsf_forward(int precision, const double *pos4, const double *box,
+                    const long long *row_off, const int *nbr, int max_row,
+                    int n_centers, int center_begin, const int *centers,
+                    const pt_sf_table *table, void *G, int ldg, pt_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * (b') ACSF adjoint.  With w = dE/dG (n_centers x ldw, same precision),
+ * writes per centre i
+ *   f_self[i]         = -sum_s w_is dG_is/dr_i     (potentials/nnp/force.py:
+ *                       16-43: the reference's gradient w.r.t. the central
+ *                       positions, neighbours frozen)
+ *   edge_grad[e(i,j)] =  sum_s w_is dG_is/dr_j     (all j in row i)
+ * f_self is FP64 (device, n_centers x 3, optional).  edge_grad is FP64 for
+ * PT_F64 and FP32 for PT_F32 (device, n_pairs x 3, optional); entries of
+ * row i beyond the descriptor cutoff are written as zero.
+ * ---------------------------------------------------------------------- */
+int pt_acsf_adjoint(int precision, const double *pos4, const double *box,
+                    const long long *row_off, const int *nbr, int max_row,
+                    int n_centers, int center_begin, const int *centers,
+                    const pt_sf_table *table, const void *w, int ldw,
+                    double *f_self, void *edge_grad, pt_stream_t stream);
+
+/* Full forces F = -grad E by a scatter-free gather over the edge buffer:
+ *   F[i] = sum_{e in row i} (edge_grad[e] - edge_grad[rev[e]]).
+ * (FP64 accumulation; edge_precision PT_F64 or PT_F32.)  Replaces the
+ * autodiff of potentials/nnp/force.py:16-43 for the -grad E force mode. */
+int pt_force_gather(int n, const long long *row_off, const int *rev,
+                    const void *edge_grad, int edge_precision, double *f_full,
+                    pt_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Descriptor tangent: T[c, s] = sum_m dG_{i,s}/dr_m . delta[m] for centre
+ * i = centers[c].  With center_only != 0 only the centre's own delta
+ * contributes (delta of neighbours taken as 0); with delta = unit vector
+ * this is one column of the self-Jacobian of descriptors/acsf/acsf.py:
+ * 88-120,215-228 (ACSF.grad).  delta: (device) n x 3 FP64, internal order.
+ * ---------------------------------------------------------------------- */
+int pt_acsf_tangent(int precision, const double *pos4, const double *box,
+                    const long long *row_off, const int *nbr, int max_row,
+                    int n_centers, int center_begin, const int *centers,
+                    const pt_sf_table *table, const double *delta, int center_only,
+                    void *T, int ldt, pt_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * (c) Atomic MLP of one element on matrix cores: scaler + dense layers +
+ * output, and the input gradient.  Replaces _compute_energy_per_atom
+ * (potentials/nnp/energy.py:23-39) = scaler (descriptors/scaler.py:206-246)
+ * + NeuralNetworkModel (models/nn/model.py:24-58), and its reverse-mode
+ * gradient w.r.t. the descriptor.
+ *  G      (device) n_atoms x ldg, FP64 or FP32 (g_precision)
+ *  scale  (device) 2 x in_dim FP64: x' = (x - scale[0,k]) * scale[1,k] + add
+ *         (every scaler kind of scaler.py:206-246 is such an affine map)
+ *  packed (device) from pt_mlp_pack
+ *  e_atom (device, out) n_atoms FP64 (model output, per atom)
+ *  dEdG   (device, out, optional) n_atoms x ldg in g_precision
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int n_layers;      /* hidden layers + the output layer (<= 4) */
+    int in_dim;        /* descriptor width (<= 128) */
+    int width[4];      /* outputs per layer; the last is 1; hidden <= 128 */
+    int act[4];        /* PT_ACT_* per layer */
+    double scale_add;  /* scaler offset: smin for the min-max kinds, else 0 */
+} pt_mlp_desc;
+
+/* Packed weights: FP64 W/b plus bf16 hi/lo copies in both W and W^T
+ * layouts, padded to multiples of 32.  params (device, FP64) holds per layer
+ * W (fan_in x fan_out, row-major, the flax kernel layout of model.py:28-38)
+ * then b (fan_out). */
+size_t pt_mlp_packed_bytes(const pt_mlp_desc *desc);
+int pt_mlp_pack(const pt_mlp_desc *desc, const double *params, void *packed,
+                pt_stream_t stream);
+
+int pt_mlp_energy_grad(int mode, int n_atoms, const void *G, int ldg, int g_precision,
+                       const double *scale, const pt_mlp_desc *desc, const void *packed,
+                       double *e_atom, void *dEdG, pt_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * (d) Reductions and the velocity-Verlet step (FP64).
+ * ---------------------------------------------------------------------- */
+size_t pt_reduce_workspace_bytes(int n);
+/* out[0] = sum x[i] in a fixed order (deterministic).  Replaces the sums of
+ * potentials/nnp/energy.py:54-63. */
+int pt_reduce_sum_f64(int n, const double *x, double *out, void *ws, size_t ws_bytes,
+                      pt_stream_t stream);
+
+/* x <- wrap(x + v dt + 1/2 a dt^2), a = F (inv_mass == NULL, the reference
+ * integrator of simulation/molecular_dynamics.py:16-20,65-72) or F/m.
+ * pos4 is updated in place (type column untouched); wrap = jnp.remainder
+ * (atoms/box.py:123-126) when box != NULL (host, 3 lengths). */
+int pt_verlet_positions(int n, double *pos4, const double *vel, const double *force,
+                        const double *inv_mass, double dt, const double *box,
+                        pt_stream_t stream);
+/* pos (n x 3, device) <- jnp.remainder(pos, box) (atoms/box.py:123-126,
+ * applied by atoms/structure.py:80-81 when a structure is created). */
+int pt_wrap_positions(int n, double *pos, const double *box, pt_stream_t stream);
+/* v <- v + 1/2 (a_old + a_new) dt (molecular_dynamics.py:23-30,73-76). */
+int pt_verlet_velocities(int n, double *vel, const double *f_old, const double *f_new,
+                         const double *inv_mass, double dt, pt_stream_t stream);
+/* out[0] = 1/2 sum m v^2 (simulation/system.py:20-22), deterministic. */
+int pt_kinetic_energy(int n, const double *vel, const double *mass, double *out,
+                      void *ws, size_t ws_bytes, pt_stream_t stream);
+/* Berendsen rescale v *= 1/sqrt(1 + dt/tau (T/T0 - 1)), T from ke[0]
+ * (simulation/thermostat.py:12-22, system.py:25-29). */
+int pt_berendsen_rescale(int n, double *vel, const double *ke, double dt, double tau,
+                         double t0, double kb, pt_stream_t stream);
+
+/* out[0] = max_i |MIC(pos4_i - ref_i)| (device scalar); drives the rebuild
+ * of skin neighbour lists (no counterpart in the reference, which has no
+ * neighbour list). */
+int pt_max_displacement(int n, const double *pos4, const double *ref, const double *box,
+                        double *out, pt_stream_t stream);
+
+/* Row gathers/scatters between input order and internal order. */
+int pt_gather_rows_f64(int n, int width, const int *idx, const double *src, double *dst,
+                       pt_stream_t stream); /* dst[i] = src[idx[i]] */
+int pt_scatter_rows_f64(int n, int width, const int *idx, const double *src, double *dst,
+                        pt_stream_t stream); /* dst[idx[i]] = src[i] */
+int pt_gather_rows_i32(int n, int width, const int *idx, const int *src, int *dst,
+                       pt_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PANTEA_HIP_H */

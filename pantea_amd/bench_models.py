@@ -1,0 +1,67 @@
+"""The water NNPs of the benchmark (teacher and student).
+
+Teacher (per central element, tanh cutoff rc = 12 Bohr):
+  * 8 G2 per neighbour element: eta log-spaced in [0.001, 0.5] Bohr^-2, r_shift 0
+  * 12 G3 per neighbour-element pair: eta in {0.001, 0.01, 0.05}, zeta in {1, 4},
+    lambda = +1/-1
+  -> 16 + 36 = 52 functions; MLP 52 -> h -> h -> 1, tanh / linear,
+  weights ~ N(0, 1/fan_in), zero biases, min-max scaler fitted on a structure.
+Student: 8 G2 per neighbour element and 6 G3 per pair (eta in {0.001, 0.01,
+0.05}, zeta 1, lambda +/-1) -> 34 functions; MLP 34 -> 16 -> 16 -> 1.
+"""
+
+from __future__ import annotations
+
+import itertools
+from typing import Sequence, Tuple
+
+import numpy as np
+
+from .descriptors import ACSF, G2, G3, CutoffFunction, DescriptorScaler, NeighborElements
+from .models import NeuralNetworkModel, NormalInitializer
+from .potential import AtomicPotential, NeuralNetworkPotential
+
+ELEMENTS = ("H", "O")
+PAIRS = (("H", "H"), ("H", "O"), ("O", "O"))
+
+
+def teacher_acsf(center: str, r_cutoff: float = 12.0, cutoff: str = "tanh") -> ACSF:
+    cfn = CutoffFunction.from_type(cutoff, r_cutoff)
+    etas = np.geomspace(0.001, 0.5, 8)
+    radial = tuple((G2(cfn, r_shift=0.0, eta=float(eta)), NeighborElements(e)) for e in ELEMENTS for eta in etas)
+    angular = tuple(
+        (G3(cfn, eta=eta, zeta=zeta, lambda0=lam, r_shift=0.0), NeighborElements(j, k))
+        for (j, k) in PAIRS for eta in (0.001, 0.01, 0.05) for zeta in (1.0, 4.0) for lam in (1.0, -1.0))
+    return ACSF(center, radial, angular)
+
+
+def student_acsf(center: str, r_cutoff: float = 12.0, cutoff: str = "tanh") -> ACSF:
+    cfn = CutoffFunction.from_type(cutoff, r_cutoff)
+    etas = np.geomspace(0.001, 0.5, 8)
+    radial = tuple((G2(cfn, r_shift=0.0, eta=float(eta)), NeighborElements(e)) for e in ELEMENTS for eta in etas)
+    angular = tuple(
+        (G3(cfn, eta=eta, zeta=1.0, lambda0=lam, r_shift=0.0), NeighborElements(j, k))
+        for (j, k) in PAIRS for eta in (0.001, 0.01, 0.05) for lam in (1.0, -1.0))
+    return ACSF(center, radial, angular)
+
+
+def make_nnp(acsf_of, hidden: Sequence[int], seed: int, precision: str = "float64",
+             scale_type: str = "scale") -> NeuralNetworkPotential:
+    aps = {}
+    params = {}
+    for i, e in enumerate(ELEMENTS):
+        model = NeuralNetworkModel(hidden_layers=tuple((h, "tanh") for h in hidden),
+                                   output_layer=(1, "identity"), kernel_initializer=NormalInitializer(1.0))
+        acsf = acsf_of(e)
+        aps[e] = AtomicPotential(acsf, DescriptorScaler.from_type(scale_type), model)
+        params[e] = model.init(seed + 101 * (i + 1), acsf.num_symmetry_functions)
+    return NeuralNetworkPotential(directory=".", elements=ELEMENTS, atomic_potentials=aps, models_params=params,
+                                  scalers_params={e: None for e in ELEMENTS}, precision=precision)
+
+
+def water_nnp(structure, hidden: Sequence[int] = (25, 25), seed: int = 2023, precision: str = "float64",
+              student: bool = False) -> NeuralNetworkPotential:
+    """Teacher (or student) NNP with its min-max scaler fitted on ``structure``."""
+    pot = make_nnp(student_acsf if student else teacher_acsf, hidden, seed, precision)
+    pot.fit_scaler([structure])
+    return pot

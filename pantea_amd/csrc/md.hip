@@ -1,0 +1,335 @@
+// (d) FP64 reductions, the velocity-Verlet step, row permutations and the
+// scatter-free force gather.
+//
+// Reductions are deterministic: fixed-size blocks reduce in a fixed tree
+// order into per-block partials, and one block sums the partials in order.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+
+#include "common.h"
+
+namespace pt {
+
+static thread_local char g_err[512];
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+size_t lds_limit() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 64 * 1024;
+    static thread_local int cached_dev = -1;
+    static thread_local size_t cached = 0;
+    if (dev != cached_dev) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 64 * 1024;
+        cached = (size_t)v;
+        cached_dev = dev;
+    }
+    return cached;
+}
+
+namespace {
+
+constexpr int kRedThreads = 256;
+constexpr int kRedItems = 4;
+constexpr int kRedTile = kRedThreads * kRedItems;
+
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double part[kRedThreads / kWave];
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if (wave_lane() == 0) part[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int q = 0; q < kRedThreads / kWave; ++q) s += part[q];
+    return s;  // valid in thread 0
+}
+
+// mode 0: x[i]; mode 1: 0.5 * m[i] * |v_i|^2 with v row-major n x 3
+template <int MODE>
+__global__ __launch_bounds__(kRedThreads) void k_reduce_partial(int n, const double* x,
+                                                                const double* m, double* partial) {
+    double s = 0.0;
+    const int base = blockIdx.x * kRedTile;
+    for (int it = 0; it < kRedItems; ++it) {
+        const int i = base + it * kRedThreads + threadIdx.x;
+        if (i < n) {
+            if (MODE == 0) s += x[i];
+            else {
+                const double vx = x[3 * i], vy = x[3 * i + 1], vz = x[3 * i + 2];
+                s += 0.5 * m[i] * ((vx * vx + vy * vy) + vz * vz);
+            }
+        }
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_reduce_final(int nb, const double* partial,
+                                                              double* out) {
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nb; k += kRedThreads) s += partial[k];
+    s = block_sum(s);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+__device__ __forceinline__ double wrap1(double x, double L) {
+    double r = fmod(x, L);
+    if (r != 0.0 && ((r < 0.0) != (L < 0.0))) r = r + L;
+    return r;
+}
+
+__global__ void k_verlet_pos(int n, double* pos4, const double* vel, const double* force,
+                             const double* inv_mass, double dt, BoxArg box) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double im = inv_mass ? inv_mass[i] : 1.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        // positions + velocities * dt + 0.5 * a * dt * dt (molecular_dynamics.py:20)
+        const double acc = inv_mass ? force[3 * i + d] * im : force[3 * i + d];
+        double x = pos4[4 * i + d] + vel[3 * i + d] * dt + 0.5 * acc * dt * dt;
+        if (box.periodic) x = wrap1(x, box.L[d]);
+        pos4[4 * i + d] = x;
+    }
+}
+
+__global__ void k_wrap(int n, double* pos, BoxArg box) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * n) return;
+    pos[t] = wrap1(pos[t], box.L[t % 3]);
+}
+
+__global__ void k_verlet_vel(int n, double* vel, const double* f_old, const double* f_new,
+                             const double* inv_mass, double dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double im = inv_mass ? inv_mass[i] : 1.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        // velocities + 0.5 * (forces + new_forces) * dt (molecular_dynamics.py:30)
+        const double ao = inv_mass ? f_old[3 * i + d] * im : f_old[3 * i + d];
+        const double an = inv_mass ? f_new[3 * i + d] * im : f_new[3 * i + d];
+        vel[3 * i + d] = vel[3 * i + d] + 0.5 * (ao + an) * dt;
+    }
+}
+
+__global__ void k_berendsen(int n, double* vel, const double* ke, double dt, double tau, double t0,
+                            double kb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double T = 2.0 * ke[0] / (3.0 * n * kb);
+    const double lam = 1.0 / sqrt(1.0 + (dt / tau) * (T / t0 - 1.0));
+    vel[3 * i] *= lam; vel[3 * i + 1] *= lam; vel[3 * i + 2] *= lam;
+}
+
+// out[0] = max_i |MIC(pos4_i - ref_i)| (one block, any n).
+__global__ __launch_bounds__(1024) void k_max_disp(int n, const double* pos4, const double* ref,
+                                                   BoxArg box, double* out) {
+    __shared__ double red[1024];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        double dx = pos4[4 * i] - ref[4 * i], dy = pos4[4 * i + 1] - ref[4 * i + 1],
+               dz = pos4[4 * i + 2] - ref[4 * i + 2];
+        if (box.periodic) {
+            dx = mic1(dx, box.L[0], box.half[0]);
+            dy = mic1(dy, box.L[1], box.half[1]);
+            dz = mic1(dz, box.L[2], box.half[2]);
+        }
+        m = fmax(m, dx * dx + dy * dy + dz * dz);
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sqrt(red[0]);
+}
+
+template <typename T>
+__global__ void k_gather(int n, int width, const int* idx, const T* src, T* dst) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * width) return;
+    const int i = (int)(t / width), k = (int)(t - (long long)i * width);
+    dst[t] = src[(long long)idx[i] * width + k];
+}
+
+template <typename T>
+__global__ void k_scatter(int n, int width, const int* idx, const T* src, T* dst) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * width) return;
+    const int i = (int)(t / width), k = (int)(t - (long long)i * width);
+    dst[(long long)idx[i] * width + k] = src[t];
+}
+
+// F[i] = sum_{e in row i} (g[e] - g[rev[e]]), one wavefront per atom.
+template <typename E>
+__global__ __launch_bounds__(256) void k_force_gather(int n, const long long* __restrict__ row_off,
+                                                      const int* __restrict__ rev,
+                                                      const E* __restrict__ g,
+                                                      double* __restrict__ f) {
+    const int lane = wave_lane();
+    const int i = uniform(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (i >= n) return;
+    const long long b = row_off[i], e = row_off[i + 1];
+    double fx = 0, fy = 0, fz = 0;
+    for (long long k = b + lane; k < e; k += kWave) {
+        const long long r = rev[k];
+        fx += (double)g[3 * k];
+        fy += (double)g[3 * k + 1];
+        fz += (double)g[3 * k + 2];
+        if (r >= 0) {  // always true for a symmetric list (checked by the tests)
+            fx -= (double)g[3 * r];
+            fy -= (double)g[3 * r + 1];
+            fz -= (double)g[3 * r + 2];
+        }
+    }
+    fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
+    if (lane == 0) { f[3 * i] = fx; f[3 * i + 1] = fy; f[3 * i + 2] = fz; }
+}
+
+inline int nblocks(long long work, int threads) { return (int)((work + threads - 1) / threads); }
+
+}  // namespace
+}  // namespace pt
+
+using namespace pt;
+
+extern "C" const char* pt_last_error(void) { return pt::g_err; }
+extern "C" int pt_version(void) { return 1; }
+
+extern "C" size_t pt_reduce_workspace_bytes(int n) {
+    const int nb = (n + kRedTile - 1) / kRedTile;
+    return sizeof(double) * (size_t)(nb > 0 ? nb : 1);
+}
+
+extern "C" int pt_reduce_sum_f64(int n, const double* x, double* out, void* ws, size_t ws_bytes,
+                                 pt_stream_t stream_) {
+    hipStream_t stream = (hipStream_t)stream_;
+    PT_REQUIRE(n >= 0 && out && ws, "pt_reduce_sum_f64: bad arguments");
+    PT_REQUIRE(ws_bytes >= pt_reduce_workspace_bytes(n), "pt_reduce_sum_f64: workspace too small");
+    const int nb = std::max(1, (n + kRedTile - 1) / kRedTile);
+    hipLaunchKernelGGL(k_reduce_partial<0>, dim3(nb), dim3(kRedThreads), 0, stream, n, x,
+                       (const double*)nullptr, (double*)ws);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, stream, nb, (const double*)ws,
+                       out);
+    return launch_check("pt_reduce_sum_f64");
+}
+
+extern "C" int pt_kinetic_energy(int n, const double* vel, const double* mass, double* out,
+                                 void* ws, size_t ws_bytes, pt_stream_t stream_) {
+    hipStream_t stream = (hipStream_t)stream_;
+    PT_REQUIRE(n >= 0 && vel && mass && out && ws, "pt_kinetic_energy: bad arguments");
+    PT_REQUIRE(ws_bytes >= pt_reduce_workspace_bytes(n), "pt_kinetic_energy: workspace too small");
+    const int nb = std::max(1, (n + kRedTile - 1) / kRedTile);
+    hipLaunchKernelGGL(k_reduce_partial<1>, dim3(nb), dim3(kRedThreads), 0, stream, n, vel, mass,
+                       (double*)ws);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, stream, nb, (const double*)ws,
+                       out);
+    return launch_check("pt_kinetic_energy");
+}
+
+extern "C" int pt_verlet_positions(int n, double* pos4, const double* vel, const double* force,
+                                   const double* inv_mass, double dt, const double* box,
+                                   pt_stream_t stream) {
+    PT_REQUIRE(n >= 0 && pos4 && vel && force, "pt_verlet_positions: bad arguments");
+    if (n == 0) return PT_OK;
+    BoxArg b{};
+    b.periodic = box != nullptr;
+    for (int d = 0; d < 3; ++d) { b.L[d] = box ? box[d] : 0; b.half[d] = 0.5 * b.L[d]; }
+    hipLaunchKernelGGL(k_verlet_pos, dim3(nblocks(n, 256)), dim3(256), 0, (hipStream_t)stream, n,
+                       pos4, vel, force, inv_mass, dt, b);
+    return launch_check("pt_verlet_positions");
+}
+
+extern "C" int pt_wrap_positions(int n, double* pos, const double* box, pt_stream_t stream) {
+    PT_REQUIRE(n >= 0 && pos && box, "pt_wrap_positions: bad arguments");
+    if (n == 0) return PT_OK;
+    BoxArg b{};
+    b.periodic = 1;
+    for (int d = 0; d < 3; ++d) { b.L[d] = box[d]; b.half[d] = 0.5 * box[d]; }
+    hipLaunchKernelGGL(k_wrap, dim3(nblocks(3LL * n, 256)), dim3(256), 0, (hipStream_t)stream, n, pos, b);
+    return launch_check("pt_wrap_positions");
+}
+
+extern "C" int pt_verlet_velocities(int n, double* vel, const double* f_old, const double* f_new,
+                                    const double* inv_mass, double dt, pt_stream_t stream) {
+    PT_REQUIRE(n >= 0 && vel && f_old && f_new, "pt_verlet_velocities: bad arguments");
+    if (n == 0) return PT_OK;
+    hipLaunchKernelGGL(k_verlet_vel, dim3(nblocks(n, 256)), dim3(256), 0, (hipStream_t)stream, n,
+                       vel, f_old, f_new, inv_mass, dt);
+    return launch_check("pt_verlet_velocities");
+}
+
+extern "C" int pt_berendsen_rescale(int n, double* vel, const double* ke, double dt, double tau,
+                                    double t0, double kb, pt_stream_t stream) {
+    PT_REQUIRE(n >= 0 && vel && ke && tau != 0.0 && t0 != 0.0, "pt_berendsen_rescale: bad arguments");
+    if (n == 0) return PT_OK;
+    hipLaunchKernelGGL(k_berendsen, dim3(nblocks(n, 256)), dim3(256), 0, (hipStream_t)stream, n,
+                       vel, ke, dt, tau, t0, kb);
+    return launch_check("pt_berendsen_rescale");
+}
+
+extern "C" int pt_max_displacement(int n, const double* pos4, const double* ref, const double* box,
+                                   double* out, pt_stream_t stream) {
+    PT_REQUIRE(n >= 0 && pos4 && ref && out, "pt_max_displacement: bad arguments");
+    BoxArg b{};
+    b.periodic = box != nullptr;
+    for (int d = 0; d < 3; ++d) { b.L[d] = box ? box[d] : 0; b.half[d] = 0.5 * b.L[d]; }
+    hipLaunchKernelGGL(k_max_disp, dim3(1), dim3(1024), 0, (hipStream_t)stream, n, pos4, ref, b, out);
+    return launch_check("pt_max_displacement");
+}
+
+extern "C" int pt_gather_rows_f64(int n, int width, const int* idx, const double* src, double* dst,
+                                  pt_stream_t stream) {
+    PT_REQUIRE(n >= 0 && width > 0 && idx && src && dst, "pt_gather_rows_f64: bad arguments");
+    if (n == 0) return PT_OK;
+    hipLaunchKernelGGL(k_gather<double>, dim3(nblocks((long long)n * width, 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, width, idx, src, dst);
+    return launch_check("pt_gather_rows_f64");
+}
+
+extern "C" int pt_scatter_rows_f64(int n, int width, const int* idx, const double* src, double* dst,
+                                   pt_stream_t stream) {
+    PT_REQUIRE(n >= 0 && width > 0 && idx && src && dst, "pt_scatter_rows_f64: bad arguments");
+    if (n == 0) return PT_OK;
+    hipLaunchKernelGGL(k_scatter<double>, dim3(nblocks((long long)n * width, 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, width, idx, src, dst);
+    return launch_check("pt_scatter_rows_f64");
+}
+
+extern "C" int pt_gather_rows_i32(int n, int width, const int* idx, const int* src, int* dst,
+                                  pt_stream_t stream) {
+    PT_REQUIRE(n >= 0 && width > 0 && idx && src && dst, "pt_gather_rows_i32: bad arguments");
+    if (n == 0) return PT_OK;
+    hipLaunchKernelGGL(k_gather<int>, dim3(nblocks((long long)n * width, 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, width, idx, src, dst);
+    return launch_check("pt_gather_rows_i32");
+}
+
+extern "C" int pt_force_gather(int n, const long long* row_off, const int* rev, const void* edge_grad,
+                               int edge_precision, double* f_full, pt_stream_t stream) {
+    PT_REQUIRE(n >= 0 && row_off && rev && edge_grad && f_full, "pt_force_gather: bad arguments");
+    if (n == 0) return PT_OK;
+    const int blocks = (n + 3) / 4;
+    if (edge_precision == PT_F64)
+        hipLaunchKernelGGL(k_force_gather<double>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, n,
+                           row_off, rev, (const double*)edge_grad, f_full);
+    else if (edge_precision == PT_F32)
+        hipLaunchKernelGGL(k_force_gather<float>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, n,
+                           row_off, rev, (const float*)edge_grad, f_full);
+    else {
+        set_error("pt_force_gather: unknown precision %d", edge_precision);
+        return PT_ERR_ARG;
+    }
+    return launch_check("pt_force_gather");
+}

@@ -1,0 +1,308 @@
+"""GPU parity: the gfx950 kernels (through the C ABI) against the FP64 oracle.
+
+Bars: neighbour lists bit-exact (same directed pairs); FP64 descriptors,
+energies and forces within 1e-10 relative (scaled by the largest magnitude
+of the compared array); FP32/BF16 paths within the error budgets stated in
+each test.
+"""
+
+import json
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, h2o_fixture_inputs
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    from pantea_amd import _lib
+
+    _lib.load()
+    return torch.device("cuda")
+
+
+def _rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _structure(data):
+    from pantea_amd.atoms import Structure
+
+    return Structure.from_dict(data)
+
+
+def _water(name):
+    from pantea_amd import water
+
+    return _structure(water.config(name))
+
+
+# ---------------------------------------------------------------------------
+# (a) neighbour list
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("case", ["cfg0_rc12", "box7_rc6", "box10_rc3", "cfg1_rc12", "cluster_nopbc"])
+def test_neighbor_list_bit_exact(dev, case):
+    from oracle import pyoracle
+    from pantea_amd import water
+    from pantea_amd.engine import NeighborList
+
+    rng = np.random.default_rng(7)
+    if case == "cfg0_rc12":
+        d, rc = water.config("cfg0"), 12.0
+    elif case == "cfg1_rc12":
+        d, rc = water.config("cfg1"), 12.0
+    elif case == "box7_rc6":
+        d = {"positions": rng.random((40, 3)) * 7.0, "elements": ["O", "H"] * 20, "lattice": np.diag([7.0] * 3)}
+        rc = 6.0
+    elif case == "box10_rc3":
+        d = {"positions": rng.random((300, 3)) * 10.0, "elements": ["H"] * 300, "lattice": np.diag([10.0] * 3)}
+        rc = 3.0
+    else:
+        d = {"positions": rng.normal(size=(150, 3)) * 4.0, "elements": ["O", "H", "H"] * 50, "lattice": []}
+        rc = 3.5
+    _log(f"neighbour case {case}")
+    s = _structure(d)
+    nl = NeighborList(rc, s.element_map.n_types).build(s.positions, s.atom_types, s.box_lengths, s.type_counts())
+    torch.cuda.synchronize()
+    counts, idx = nl.to_csr_input_order()
+    pos = s.positions.cpu().numpy()
+    oc, oi = pyoracle.neighbors(pos, s.box_lengths, rc)
+    assert np.array_equal(counts, oc)
+    assert np.array_equal(idx, oi)
+    # reverse edges
+    rev = nl.rev[: nl.n_pairs].cpu().numpy()
+    nbr = nl.nbr[: nl.n_pairs].cpu().numpy()
+    off = nl.row_off.cpu().numpy()
+    assert np.all(rev >= 0)
+    assert np.array_equal(rev[rev], np.arange(nl.n_pairs))
+    rows_of = np.repeat(np.arange(nl.n), np.diff(off))
+    assert np.array_equal(nbr[rev], rows_of)
+    # internal order groups types
+    t_int = s.atom_types_host[nl.perm_host]
+    assert np.all(np.diff(t_int) >= 0)
+
+
+# ---------------------------------------------------------------------------
+# (b) descriptors
+# ---------------------------------------------------------------------------
+
+def test_acsf_goldens_through_api(dev, goldens):
+    from pantea_amd.descriptors import ACSF, G2, G3, CutoffFunction, NeighborElements
+
+    g = goldens["acsf_without_pbc"]
+    cfn = CutoffFunction.from_type("tanhu", 3.0)
+    acsf = ACSF("Ne", tuple((G2(cfn, r_shift=rs, eta=eta), NeighborElements("Ne")) for eta, rs in g["g2"]), ())
+    s = _structure({"positions": g["positions"], "elements": g["elements"]})
+    G = acsf(s).cpu().numpy()
+    assert G.shape == (2, 5)
+    assert np.max(np.abs(G - np.tile(g["expected_row"], (2, 1)))) < 1e-10
+
+    g = goldens["acsf_with_pbc"]
+    cfn = CutoffFunction.from_type("tanhu", g["cutoff"][1])
+    acsf = ACSF("O", ((G2(cfn, r_shift=0.0, eta=0.001), NeighborElements("H")),),
+                ((G3(cfn, eta=0.07, zeta=1.0, lambda0=1.0, r_shift=0.0), NeighborElements("H", "H")),))
+    s = _structure({"positions": g["positions"], "elements": g["elements"], "lattice": g["lattice"]})
+    assert tuple(acsf(s).shape) == (4, 2)
+    row = acsf(s, atom_index=0).cpu().numpy()[0]
+    assert np.max(np.abs(row - np.array(g["expected_row"]))) < 1e-10
+    with pytest.raises(ValueError):
+        acsf(s, atom_index=1)  # an H atom for an O descriptor (acsf.py:70-80)
+
+
+@pytest.mark.parametrize("cutoff", ["tanh", "cos"])
+def test_acsf_fp64_fp32_vs_oracle(dev, cutoff):
+    from oracle.bridge import sf_rows
+    from oracle import pyoracle
+    from pantea_amd.bench_models import teacher_acsf
+
+    s = _water("cfg0")
+    pos = s.positions.cpu().numpy()
+    emap = s.element_map.element_to_atom_type
+    for el in ("O", "H"):
+        acsf = teacher_acsf(el, cutoff=cutoff)
+        _log(f"acsf {el} {cutoff}")
+        idx = s.select_host(el)
+        ref = pyoracle.acsf(pos, s.atom_types_host, s.box_lengths, sf_rows(acsf, emap), center_pos=pos[idx],
+                           self_idx=idx.astype(np.int32))
+        g64 = acsf(s).cpu().numpy()
+        assert _rel(g64, ref) < 1e-10
+        g32 = acsf(s, precision="float32").cpu().numpy()
+        assert _rel(g32, ref) < 2e-5
+
+
+def test_acsf_grad_vs_oracle(dev):
+    from oracle.bridge import sf_rows
+    from oracle import pyoracle
+    from pantea_amd.bench_models import teacher_acsf
+
+    s = _water("cfg0")
+    pos = s.positions.cpu().numpy()
+    acsf = teacher_acsf("O")
+    rows = sf_rows(acsf, s.element_map.element_to_atom_type)
+    idx = np.arange(0, s.natoms, 7)  # atoms of both elements (acsf.py:103-104 semantics)
+    J = acsf.grad(s, atom_index=idx).cpu().numpy()
+    ref = pyoracle.acsf_selfjac(pos, s.atom_types_host, s.box_lengths, rows, center_pos=pos[idx],
+                                self_idx=idx.astype(np.int32))
+    assert J.shape == ref.shape
+    assert _rel(J, ref) < 1e-10
+    J32 = acsf.grad(s, atom_index=idx, precision="float32").cpu().numpy()
+    assert _rel(J32, ref) < 1e-4
+
+
+# ---------------------------------------------------------------------------
+# (c) MLP on matrix cores
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("mode,tol", [("float64", 1e-12), ("bf16x3", 2e-4), ("bf16", 3e-2)])
+@pytest.mark.parametrize("shape", [(52, (25, 25)), (52, (64, 64)), (34, (16, 16)), (3, (5, 5)), (70, (100, 40, 8))])
+def test_mlp_vs_oracle(dev, mode, tol, shape):
+    from oracle import pyoracle
+    from pantea_amd import _lib
+    from pantea_amd.models import MLP_MODES, DeviceMLP, NeuralNetworkModel, NormalInitializer
+
+    n_in, hidden = shape
+    acts = ["tanh", "softplus", "logistic"][: len(hidden)]
+    model = NeuralNetworkModel(tuple(zip(hidden, acts)), (1, "identity"), kernel_initializer=NormalInitializer())
+    params = model.init(11, n_in)
+    for k in params:
+        params[k]["bias"] = np.random.default_rng(3).normal(size=params[k]["bias"].shape) * 0.1
+    n = 203
+    rng = np.random.default_rng(5)
+    G = rng.random((n, n_in)) * 2.0
+    sub, mul = rng.random(n_in), 0.5 + rng.random(n_in)
+    add = 0.25
+    gdt = torch.float64 if mode == "float64" else torch.float32
+    Gd = torch.as_tensor(G, dtype=gdt, device=dev)
+    mlp = DeviceMLP(model, params, n_in, np.stack([sub, mul]), add, dev)
+    e = torch.zeros(n, dtype=torch.float64, device=dev)
+    dE = torch.zeros((n, n_in), dtype=gdt, device=dev)
+    mlp.energy_grad(MLP_MODES[mode], n, Gd, n_in, _lib.PT_F64 if mode == "float64" else _lib.PT_F32, e, dE)
+    torch.cuda.synchronize()
+    layers = [(params[f"layers_{2*i}"]["kernel"], params[f"layers_{2*i}"]["bias"], a)
+              for i, a in enumerate(acts + ["identity"])]
+    x = (G - sub) * mul + add
+    y, g = pyoracle.mlp_forward_backward(layers, x)
+    g = g * mul
+    assert _rel(e.cpu().numpy(), y) < tol
+    assert _rel(dE.cpu().numpy(), g) < tol
+
+
+# ---------------------------------------------------------------------------
+# NNP energy and forces
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def cfg0_teacher(dev):
+    from oracle.bridge import oracle_nnp
+    from pantea_amd import bench_models
+
+    s = _water("cfg0")
+    pot = bench_models.water_nnp(s, hidden=(25, 25), seed=2023)
+    onn = oracle_nnp(pot, s)
+    ref = onn.energy_forces(s.positions.cpu().numpy(), s.atom_types_host, s.box_lengths)
+    return s, pot, onn, ref
+
+
+def test_nnp_fp64_vs_oracle(cfg0_teacher):
+    s, pot, onn, (E, e_atom, f_self, f_full) = cfg0_teacher
+    e_gpu, ff = pot.energy_and_forces(s, mode="full")
+    _, fs = pot.energy_and_forces(s, mode="reference")
+    ea = pot.atomic_energies(s).cpu().numpy()
+    _log(f"E gpu={float(e_gpu):.15f} oracle={E:.15f}")
+    assert abs(float(e_gpu) - E) <= 1e-10 * np.sum(np.abs(e_atom))
+    assert _rel(ea, e_atom) < 1e-10
+    assert _rel(fs.cpu().numpy(), f_self) < 1e-10
+    assert _rel(ff.cpu().numpy(), f_full) < 1e-10
+    assert np.abs(ff.cpu().numpy().sum(0)).max() < 1e-9 * np.abs(f_full).max()
+    # compute_forces keeps the reference semantics by default
+    assert _rel(pot.compute_forces(s).cpu().numpy(), f_self) < 1e-10
+
+
+def test_nnp_mixed_precision_error(cfg0_teacher):
+    from pantea_amd.units import units
+
+    s, pot, onn, (E, e_atom, f_self, f_full) = cfg0_teacher
+    for prec, f_budget in (("mixed", 5.0), ("mixed_bf16", None)):
+        e_m, f_m = pot.energy_and_forces(s, mode="full", precision=prec)
+        de = abs(float(e_m) - E) / s.natoms * units.TO_MILLI_ELECTRON_VOLT
+        rmse = np.sqrt(np.mean((f_m.cpu().numpy() - f_full) ** 2)) * units.TO_MILLI_ELECTRON_VOLT
+        _log(f"{prec}: energy error {de:.4f} meV/atom, force RMSE {rmse:.4f} meV/Bohr "
+             f"(|F| rms {np.sqrt(np.mean(f_full**2)) * units.TO_MILLI_ELECTRON_VOLT:.1f} meV/Bohr)")
+        if f_budget is not None:
+            assert rmse < f_budget
+
+
+def test_h2o_fixture_golden_through_api(dev, goldens):
+    """test_nnp.py end to end through NeuralNetworkPotential.from_runner + load()."""
+    from pantea_amd.io import Dataset
+    from pantea_amd.potential import NeuralNetworkPotential
+
+    g = goldens["nnp_outputs"]
+    pot = NeuralNetworkPotential.from_runner(GOLDEN / "h2o.json")
+    assert pot.num_elements == 2 and pot.elements == ("H", "O")
+    pot.load_scaler()
+    pot.load_model()
+    s = Dataset.from_runner(GOLDEN / "h2o.data")[0]
+    F = pot.compute_forces(s).cpu().numpy()
+    assert np.allclose(F, np.array(g["forces"]), rtol=1e-5, atol=1e-8)
+    assert abs(float(pot(s)) - g["energy"]) < 2.5e-7
+    pos, types, L, onn = h2o_fixture_inputs(np.float64)
+    E, _, fs, _ = onn.energy_forces(s.positions.cpu().numpy(), s.atom_types_host, s.box_lengths)
+    assert _rel(F, fs) < 1e-10
+
+
+def test_cfg1_subset_and_fd(dev):
+    """12,288 atoms: forces on 64 atoms vs oracle (halo-restricted) and vs central FD (h = 1e-5)."""
+    from oracle.bridge import oracle_nnp
+    from pantea_amd import bench_models
+    from pantea_amd.potential import energy
+
+    s = _water("cfg1")
+    pot = bench_models.water_nnp(s, hidden=(25, 25), seed=2023)
+    rng = np.random.default_rng(64)
+    atoms = rng.choice(s.natoms, 64, replace=False)
+    x = s.positions.clone().requires_grad_(True)
+    e = energy(pot, s, x)
+    (grad,) = torch.autograd.grad(e, x)
+    f = -grad.cpu().numpy()
+    # oracle: only centres within rc of the checked atoms contribute to their forces
+    pos = s.positions.cpu().numpy()
+    from oracle import pyoracle
+
+    want = np.zeros(s.natoms, np.uint8)
+    counts, idx = pyoracle.neighbors(pos, s.box_lengths, 12.0)
+    rows = np.split(idx, np.cumsum(counts)[:-1])
+    for a in atoms:
+        want[a] = 1
+        want[rows[a]] = 1
+    onn = oracle_nnp(pot, s)
+    e_atom, w, _ = onn.atomic(pos, s.atom_types_host, s.box_lengths)
+    _, f_full = pyoracle.force_vjp(pos, s.atom_types_host, s.box_lengths, onn._tables_by_type(), w, want)
+    assert _rel(f[atoms], f_full[atoms]) < 1e-10
+    h = 1e-5
+    worst = 0.0
+    for a in atoms[:16]:
+        for d in range(3):
+            p = s.positions.clone(); p[a, d] += h
+            m = s.positions.clone(); m[a, d] -= h
+            sp, sm = s.copy(), s.copy()
+            sp.positions, sm.positions = p, m
+            fd = -(float(pot(sp)) - float(pot(sm))) / (2 * h)
+            worst = max(worst, abs(fd - f[a, d]) / max(1e-3, abs(f[a, d])))
+    _log(f"cfg1 FD worst relative deviation {worst:.2e}")
+    assert worst < 1e-5

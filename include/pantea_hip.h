@@ -226,6 +226,33 @@ int pt_mlp_energy_grad(int mode, int n_atoms, const void *G, int ldg, int g_prec
                        double *e_atom, void *dEdG, pt_stream_t stream);
 
 /* ------------------------------------------------------------------------
+ * (c') Training of an atomic MLP (the student surrogate).  No counterpart in
+ * the reference runs (its trainer, potentials/nnp/gradient_descent.py:
+ * 157-233, is broken); the loss follows BASELINE: w_E ((E - E*)/N)^2 +
+ * w_F / (3N) sum |F - F*|^2 per structure.
+ * pt_mlp_train_grad accumulates into grad (FP64, layout of the unpacked
+ * params) the gradient  sum_i gy[i] dy_i/dtheta + tseed d(ydot_i)/dtheta,
+ * y = MLP(scaled G), ydot = grad_G y . T (T: descriptor tangent).  With
+ * gy = dL/dE and T = J delta, delta = dL/dF, tseed = -1, this is dL/dtheta.
+ * ---------------------------------------------------------------------- */
+size_t pt_mlp_param_count(const pt_mlp_desc *desc);
+int pt_mlp_train_grad(int mode, int n_atoms, const void *G, const void *T, int ldg,
+                      int g_precision, const double *scale, const pt_mlp_desc *desc,
+                      const void *packed, const double *gy, double tseed, double *grad,
+                      pt_stream_t stream);
+/* delta = w_F 2 (F - F*) / (3N), gy = w_E 2 (E - E*) / N^2 (per atom), and
+ * loss[0] += w_E ((E - E*)/N)^2, loss[1] += w_F / (3N) sum |F - F*|^2.
+ * energy: device scalar. */
+int pt_train_seeds(int n, const double *force, const double *force_ref, const double *energy,
+                   double energy_ref, double w_energy, double w_force, double *delta, double *gy,
+                   double *loss, pt_stream_t stream);
+/* Adam (decoupled weight decay when weight_decay > 0), step counts from 1;
+ * optax.adamw semantics (gradient_descent.py:70-76). */
+int pt_adam_step(int n, double *params, const double *grad, double *m, double *v, double lr,
+                 double beta1, double beta2, double eps, double weight_decay, int step,
+                 pt_stream_t stream);
+
+/* ------------------------------------------------------------------------
  * (d) Reductions and the velocity-Verlet step (FP64).
  * ---------------------------------------------------------------------- */
 size_t pt_reduce_workspace_bytes(int n);
@@ -255,9 +282,9 @@ int pt_kinetic_energy(int n, const double *vel, const double *mass, double *out,
 int pt_berendsen_rescale(int n, double *vel, const double *ke, double dt, double tau,
                          double t0, double kb, pt_stream_t stream);
 
-/* out[0] = max_i |MIC(pos4_i - ref_i)| (device scalar); drives the rebuild
- * of skin neighbour lists (no counterpart in the reference, which has no
- * neighbour list). */
+/* out[0] = max_i |MIC(pos4_i - ref_i)| (device scalar; out holds 257
+ * doubles, out[1..256] are scratch); drives the rebuild of skin neighbour
+ * lists (no counterpart in the reference, which has no neighbour list). */
 int pt_max_displacement(int n, const double *pos4, const double *ref, const double *box,
                         double *out, pt_stream_t stream);
 

@@ -117,6 +117,12 @@ _SIGS = {
     "pt_gather_rows_i32": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
     "pt_wrap_positions": (_c_int, [_c_int, _vp, _vp, _vp]),
     "pt_max_displacement": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
+    "pt_mlp_param_count": (_c_size, [_vp]),
+    "pt_mlp_train_grad": (_c_int, [_c_int, _c_int, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp, _vp,
+                                   _c_dbl, _vp, _vp]),
+    "pt_train_seeds": (_c_int, [_c_int, _vp, _vp, _vp, _c_dbl, _c_dbl, _c_dbl, _vp, _vp, _vp, _vp]),
+    "pt_adam_step": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _c_dbl,
+                              _c_int, _vp]),
 }
 
 #: Every symbol the header declares; the CPU test suite checks each is exported.

@@ -131,12 +131,13 @@ __global__ void k_berendsen(int n, double* vel, const double* ke, double dt, dou
     vel[3 * i] *= lam; vel[3 * i + 1] *= lam; vel[3 * i + 2] *= lam;
 }
 
-// out[0] = max_i |MIC(pos4_i - ref_i)| (one block, any n).
-__global__ __launch_bounds__(1024) void k_max_disp(int n, const double* pos4, const double* ref,
-                                                   BoxArg box, double* out) {
-    __shared__ double red[1024];
+// Squared displacement maxima: one partial per block, then one block over
+// the partials; out[0] = max_i |MIC(pos4_i - ref_i)|.
+__global__ __launch_bounds__(256) void k_max_disp_partial(int n, const double* pos4, const double* ref,
+                                                          BoxArg box, double* partial) {
+    __shared__ double red[256];
     double m = 0.0;
-    for (int i = threadIdx.x; i < n; i += 1024) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         double dx = pos4[4 * i] - ref[4 * i], dy = pos4[4 * i + 1] - ref[4 * i + 1],
                dz = pos4[4 * i + 2] - ref[4 * i + 2];
         if (box.periodic) {
@@ -148,7 +149,20 @@ __global__ __launch_bounds__(1024) void k_max_disp(int n, const double* pos4, co
     }
     red[threadIdx.x] = m;
     __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_max_disp_final(int nb, const double* partial, double* out) {
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int k = threadIdx.x; k < nb; k += 256) m = fmax(m, partial[k]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
         __syncthreads();
     }
@@ -285,7 +299,12 @@ extern "C" int pt_max_displacement(int n, const double* pos4, const double* ref,
     BoxArg b{};
     b.periodic = box != nullptr;
     for (int d = 0; d < 3; ++d) { b.L[d] = box ? box[d] : 0; b.half[d] = 0.5 * b.L[d]; }
-    hipLaunchKernelGGL(k_max_disp, dim3(1), dim3(1024), 0, (hipStream_t)stream, n, pos4, ref, b, out);
+    // out[1 .. 1 + kDispBlocks) holds the block partials (out must hold 1 + 256 doubles)
+    const int nb = std::max(1, std::min(256, (n + 255) / 256));
+    hipLaunchKernelGGL(k_max_disp_partial, dim3(nb), dim3(256), 0, (hipStream_t)stream, n, pos4, ref, b,
+                       out + 1);
+    hipLaunchKernelGGL(k_max_disp_final, dim3(1), dim3(256), 0, (hipStream_t)stream, nb,
+                       (const double*)(out + 1), out);
     return launch_check("pt_max_displacement");
 }
 

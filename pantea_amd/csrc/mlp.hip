@@ -140,6 +140,31 @@ __device__ __forceinline__ void activate(int a, T z, T& y, T& dy) {
     y = z; dy = T(1);
 }
 
+// y, dy/dz and d2y/dz2 (the training kernel differentiates the input gradient)
+template <typename T>
+__device__ __forceinline__ void activate2(int a, T z, T& y, T& dy, T& ddy) {
+    switch (a) {
+    case PT_ACT_IDENTITY: y = z; dy = T(1); ddy = T(0); return;
+    case PT_ACT_TANH: { T t = tanh(z); y = t; dy = T(1) - t * t; ddy = T(-2) * t * dy; return; }
+    case PT_ACT_LOGISTIC: {
+        T s = T(1) / (T(1) + exp(-z)); y = s; dy = s * (T(1) - s); ddy = dy * (T(1) - T(2) * s); return;
+    }
+    case PT_ACT_SOFTPLUS: {
+        T az = z > T(0) ? z : -z;
+        y = (z > T(0) ? z : T(0)) + log1p(exp(-az));
+        T s = T(1) / (T(1) + exp(-z));
+        dy = s; ddy = s * (T(1) - s);
+        return;
+    }
+    case PT_ACT_RELU: y = z > T(0) ? z : T(0); dy = z > T(0) ? T(1) : T(0); ddy = T(0); return;
+    case PT_ACT_GAUSSIAN: { T g = exp(T(-0.5) * z * z); y = g; dy = -z * g; ddy = (z * z - T(1)) * g; return; }
+    case PT_ACT_COS: y = cos(z); dy = -sin(z); ddy = -y; return;
+    case PT_ACT_EXP: { T e = exp(-z); y = e; dy = -e; ddy = e; return; }
+    case PT_ACT_HARMONIC: y = z * z; dy = T(2) * z; ddy = T(2); return;
+    }
+    y = z; dy = T(1); ddy = T(0);
+}
+
 __device__ __forceinline__ unsigned short bf16_bits(float x) {
     // round to nearest even
     unsigned u = __float_as_uint(x);
@@ -410,6 +435,314 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
     }
 }
 
+// ---- training: forward-over-reverse through the MLP -------------------------
+//
+// Per atom the loss gradient w.r.t. the weights is
+//   gy * dy/dtheta + tseed * d(ydot)/dtheta,
+// y = MLP(x'), ydot = grad_x' MLP . xdot' (xdot' = T * mul, the descriptor
+// tangent in scaled space).  Rows 0..15 of a 32-row tile carry the 16 atoms'
+// primal values, rows 16..31 their tangents, so every layer (and its reverse)
+// is one GEMM over 32 rows on the matrix cores; the weight gradients are
+// GEMMs over those 32 rows (K = atoms x {primal, tangent}).
+
+// C(16 x 16) = A^T B over the 32 rows of two LDS tiles: A[r][i0 + ..], B[r][o0 + ..].
+template <int MODE>
+__device__ __forceinline__ void wgrad_tile(const typename MT<MODE>::T* A, const typename MT<MODE>::T* B,
+                                           int stride, int i0, int o0, typename MT<MODE>::T out[4]) {
+    const int l = wave_lane();
+    const int row = l & 15, grp = l >> 4;
+    if constexpr (MODE == PT_MLP_F64) {
+        pt_d4 c = {0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < 32; k0 += 4) {
+            const int k = k0 + grp;
+            c = __builtin_amdgcn_mfma_f64_16x16x4f64(A[k * stride + i0 + row], B[k * stride + o0 + row], c, 0, 0, 0);
+        }
+        out[0] = c[0]; out[1] = c[1]; out[2] = c[2]; out[3] = c[3];
+    } else {
+        pt_f4 c = {0.f, 0.f, 0.f, 0.f};
+        pt_bf8 ah, al, bh, bl;
+        const int kb = 8 * grp;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            unsigned short h1, l1, h2, l2;
+            split_bf16(A[(kb + q) * stride + i0 + row], h1, l1);
+            split_bf16(B[(kb + q) * stride + o0 + row], h2, l2);
+            ah[q] = __builtin_bit_cast(__bf16, h1);
+            al[q] = __builtin_bit_cast(__bf16, l1);
+            bh[q] = __builtin_bit_cast(__bf16, h2);
+            bl[q] = __builtin_bit_cast(__bf16, l2);
+        }
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
+        if (MODE == PT_MLP_BF16X3) {
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
+        }
+        out[0] = c[0]; out[1] = c[1]; out[2] = c[2]; out[3] = c[3];
+    }
+}
+
+struct TrainArgs {
+    MlpShape s;
+    int n_atoms;
+    const void* G;
+    const void* T;
+    int ldg;
+    int g_f64;
+    const double* scale;
+    double scale_add;
+    const char* packed;
+    const double* gy;     // per-atom seed of y
+    double tseed;         // seed of ydot (same for every atom)
+    double* grad;         // parameter gradient, FP64, layout of the unpacked params
+    int stride;
+    int nparam;
+};
+
+__device__ __forceinline__ void wave_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_mlp_train(TrainArgs a) {
+    typedef typename MT<MODE>::T T;
+    extern __shared__ __align__(16) char smem[];
+    const MlpShape& s = a.s;
+    const int H = s.n_hidden;
+    const int l = wave_lane();
+    const int wv = threadIdx.x >> 6;
+    const int wpb = blockDim.x >> 6;
+    const int stride = a.stride;
+    const size_t tile = (size_t)32 * stride;
+    const size_t per_wave = (size_t)(2 * H + 3) * tile + 64 + a.nparam;
+    T* base = reinterpret_cast<T*>(smem) + (size_t)wv * per_wave;
+    T* Abuf[kMaxHidden + 1];
+    T* Zbuf[kMaxHidden];
+    for (int h = 0; h <= H; ++h) Abuf[h] = base + h * tile;
+    for (int h = 0; h < H; ++h) Zbuf[h] = base + (H + 1 + h) * tile;
+    T* P[2] = {base + (2 * H + 1) * tile, base + (2 * H + 2) * tile};
+    T* outv = base + (2 * H + 3) * tile;  // [0,16): z_L, [16,32): zdot_L, [32,48): zbar_L, [48,64): zdotbar_L
+    T* gacc = outv + 64;
+    for (int k = l; k < a.nparam; k += kWave) gacc[k] = T(0);
+    const int row = l & 15, grp = l >> 4;
+    const T* wout = (MODE == PT_MLP_F64) ? (const T*)(a.packed + s.wout64) : (const T*)(a.packed + s.wout32);
+    const T bout = (MODE == PT_MLP_F64) ? (T)((const double*)(a.packed + s.bout64))[0]
+                                        : (T)((const float*)(a.packed + s.bout32))[0];
+    const int ntiles = (a.n_atoms + 15) / 16;
+    for (int tl = blockIdx.x * wpb + wv; tl < ntiles; tl += gridDim.x * wpb) {
+        const int a0 = tl * 16;
+        // scaled input and its tangent
+        for (int e = l; e < 16 * s.in_p; e += kWave) {
+            const int r = e / s.in_p, k = e % s.in_p;
+            const int atom = a0 + r;
+            T v = T(0), t = T(0);
+            if (atom < a.n_atoms && k < s.in_dim) {
+                const size_t o = (size_t)atom * a.ldg + k;
+                const double g = a.g_f64 ? ((const double*)a.G)[o] : (double)((const float*)a.G)[o];
+                const double gt = a.g_f64 ? ((const double*)a.T)[o] : (double)((const float*)a.T)[o];
+                v = (T)((g - a.scale[k]) * a.scale[s.in_dim + k] + a.scale_add);
+                t = (T)(gt * a.scale[s.in_dim + k]);
+            }
+            Abuf[0][r * stride + k] = v;
+            Abuf[0][(16 + r) * stride + k] = t;
+        }
+        wave_fence();
+        // forward: primal rows 0..15, tangent rows 16..31
+        int kin = s.in_p;
+        for (int h = 0; h < H; ++h) {
+            const int wp = s.wp[h];
+            for (int n0 = 0; n0 < wp; n0 += 16) {
+                T az[4] = {T(0), T(0), T(0), T(0)}, at[4] = {T(0), T(0), T(0), T(0)};
+                if (MODE == PT_MLP_F64) {
+                    gemm_tile<MODE>(Abuf[h], stride, kin, a.packed, s.W64[h], 0, wp, false, n0, az);
+                    gemm_tile<MODE>(Abuf[h] + 16 * stride, stride, kin, a.packed, s.W64[h], 0, wp, false, n0, at);
+                } else {
+                    gemm_tile<MODE>(Abuf[h], stride, kin, a.packed, s.WTh[h], s.WTl[h], kin, true, n0, az);
+                    gemm_tile<MODE>(Abuf[h] + 16 * stride, stride, kin, a.packed, s.WTh[h], s.WTl[h], kin, true, n0, at);
+                }
+                const int col = n0 + row;
+                const T bias = (MODE == PT_MLP_F64) ? (T)((const double*)(a.packed + s.b64[h]))[col]
+                                                    : (T)((const float*)(a.packed + s.b32[h]))[col];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int r = crow<MODE>(grp, i);
+                    const T z = az[i] + bias, zd = at[i];
+                    T y, dy;
+                    activate<T>(s.act[h], z, y, dy);
+                    const bool pad = col >= s.w[h];
+                    Zbuf[h][r * stride + col] = pad ? T(0) : z;
+                    Zbuf[h][(16 + r) * stride + col] = pad ? T(0) : zd;
+                    Abuf[h + 1][r * stride + col] = pad ? T(0) : y;
+                    Abuf[h + 1][(16 + r) * stride + col] = pad ? T(0) : dy * zd;
+                }
+            }
+            wave_fence();
+            kin = wp;
+        }
+        // output layer and its reverse seeds: 4 lanes per atom
+        {
+            const int r = l >> 2, part = l & 3;
+            T z = T(0), zd = T(0);
+            for (int k = part; k < kin; k += 4) {
+                z += Abuf[H][r * stride + k] * wout[k];
+                zd += Abuf[H][(16 + r) * stride + k] * wout[k];
+            }
+            z += __shfl_xor(z, 1, kWave); z += __shfl_xor(z, 2, kWave);
+            zd += __shfl_xor(zd, 1, kWave); zd += __shfl_xor(zd, 2, kWave);
+            z += bout;
+            T y, dy, ddy;
+            activate2<T>(s.act_out, z, y, dy, ddy);
+            const int atom = a0 + r;
+            const bool ok = atom < a.n_atoms;
+            const T gy = ok ? (T)a.gy[atom] : T(0);
+            const T gt = ok ? (T)a.tseed : T(0);
+            if (part == 0) {
+                outv[r] = z;
+                outv[16 + r] = zd;
+                outv[32 + r] = gy * dy + gt * ddy * zd;  // zbar_L
+                outv[48 + r] = gt * dy;                  // zdotbar_L
+            }
+        }
+        wave_fence();
+        // gradients of the output layer
+        const int L = H;
+        for (int k = l; k < kin; k += kWave) {
+            T gw = T(0);
+            for (int r = 0; r < 16; ++r)
+                gw += Abuf[H][r * stride + k] * outv[32 + r] + Abuf[H][(16 + r) * stride + k] * outv[48 + r];
+            const int fan_in = H > 0 ? s.w[H - 1] : s.in_dim;
+            if (k < fan_in) gacc[s.pW[L] + k] += gw;
+        }
+        if (l == 0) {
+            T gb = T(0);
+            for (int r = 0; r < 16; ++r) gb += outv[32 + r];
+            gacc[s.pb[L]] += gb;
+        }
+        // abar, adotbar of the last hidden output -> P[0]
+        for (int e = l; e < 16 * kin; e += kWave) {
+            const int r = e / kin, k = e % kin;
+            P[0][r * stride + k] = outv[32 + r] * wout[k];
+            P[0][(16 + r) * stride + k] = outv[48 + r] * wout[k];
+        }
+        wave_fence();
+        int cur = 0;
+        for (int h = H - 1; h >= 0; --h) {
+            const int wp = s.wp[h];
+            const int kin_h = h > 0 ? s.wp[h - 1] : s.in_p;
+            // zbar = abar f'(z) + adotbar f''(z) zdot ; zdotbar = adotbar f'(z)   (in place)
+            for (int e = l; e < 16 * wp; e += kWave) {
+                const int r = e / wp, k = e % wp;
+                const T z = Zbuf[h][r * stride + k], zd = Zbuf[h][(16 + r) * stride + k];
+                T y, dy, ddy;
+                activate2<T>(s.act[h], z, y, dy, ddy);
+                const T ab = P[cur][r * stride + k], adb = P[cur][(16 + r) * stride + k];
+                const bool pad = k >= s.w[h];
+                P[cur][r * stride + k] = pad ? T(0) : ab * dy + adb * ddy * zd;
+                P[cur][(16 + r) * stride + k] = pad ? T(0) : adb * dy;
+            }
+            wave_fence();
+            // dW_h += A_h^T [zbar; zdotbar]  (K = 32 rows)
+            const int fan_in = h > 0 ? s.w[h - 1] : s.in_dim;
+            for (int i0 = 0; i0 < kin_h; i0 += 16) {
+                for (int o0 = 0; o0 < wp; o0 += 16) {
+                    T c[4];
+                    wgrad_tile<MODE>(Abuf[h], P[cur], stride, i0, o0, c);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int in = i0 + crow<MODE>(grp, i), out = o0 + row;
+                        if (in < fan_in && out < s.w[h]) gacc[s.pW[h] + (size_t)in * s.w[h] + out] += c[i];
+                    }
+                }
+            }
+            for (int k = l; k < s.w[h]; k += kWave) {
+                T gb = T(0);
+                for (int r = 0; r < 16; ++r) gb += P[cur][r * stride + k];
+                gacc[s.pb[h] + k] += gb;
+            }
+            if (h > 0) {
+                // [abar; adotbar] of layer h's input = [zbar; zdotbar] W_h^T  -> P[cur ^ 1]
+                for (int n0 = 0; n0 < kin_h; n0 += 16) {
+                    T c0[4] = {T(0), T(0), T(0), T(0)}, c1[4] = {T(0), T(0), T(0), T(0)};
+                    if (MODE == PT_MLP_F64) {
+                        gemm_tile<MODE>(P[cur], stride, wp, a.packed, s.W64[h], 0, wp, true, n0, c0);
+                        gemm_tile<MODE>(P[cur] + 16 * stride, stride, wp, a.packed, s.W64[h], 0, wp, true, n0, c1);
+                    } else {
+                        gemm_tile<MODE>(P[cur], stride, wp, a.packed, s.Wh[h], s.Wl[h], wp, true, n0, c0);
+                        gemm_tile<MODE>(P[cur] + 16 * stride, stride, wp, a.packed, s.Wh[h], s.Wl[h], wp, true, n0, c1);
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int r = crow<MODE>(grp, i);
+                        P[cur ^ 1][r * stride + n0 + row] = c0[i];
+                        P[cur ^ 1][(16 + r) * stride + n0 + row] = c1[i];
+                    }
+                }
+                cur ^= 1;
+            }
+            wave_fence();
+        }
+    }
+    wave_fence();
+    for (int k = l; k < a.nparam; k += kWave) {
+        const double v = (double)gacc[k];
+        if (v != 0.0) atomicAdd(&a.grad[k], v);
+    }
+}
+
+template <int MODE>
+int launch_train(TrainArgs a, hipStream_t st) {
+    typedef typename MT<MODE>::T T;
+    const int H = a.s.n_hidden;
+    a.stride = a.s.wmax + (sizeof(T) == 8 ? 1 : 4);
+    const size_t per_wave = sizeof(T) * ((size_t)(2 * H + 3) * 32 * a.stride + 64 + a.nparam);
+    const size_t budget = lds_limit();
+    if (per_wave > budget) {
+        set_error("mlp training: %zu bytes of LDS per wavefront exceed the device limit", per_wave);
+        return PT_ERR_UNSUPPORTED;
+    }
+    const int wpb = (int)std::min<size_t>(4, budget / per_wave);
+    const int ntiles = (a.n_atoms + 15) / 16;
+    const int blocks = std::max(1, std::min((ntiles + wpb - 1) / wpb, 1024));
+    hipLaunchKernelGGL(k_mlp_train<MODE>, dim3(blocks), dim3(64 * wpb), wpb * per_wave, st, a);
+    return launch_check("k_mlp_train");
+}
+
+// ---- loss seeds and Adam --------------------------------------------------------
+
+__global__ void k_train_seeds(int n, const double* __restrict__ f, const double* __restrict__ fref,
+                              const double* __restrict__ energy, double eref, double we, double wf,
+                              double* __restrict__ delta, double* __restrict__ gy, double* __restrict__ loss) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double de = energy[0] - eref;
+    double s = 0.0;
+    if (i < n) {
+        for (int d = 0; d < 3; ++d) {
+            const double r = f[3 * i + d] - fref[3 * i + d];
+            delta[3 * i + d] = wf * 2.0 * r / (3.0 * n);
+            s += r * r;
+        }
+        gy[i] = we * 2.0 * de / ((double)n * n);
+    }
+    s = wave_sum(s);  // all lanes of the (full) block take part
+    if (wave_lane() == 0 && s != 0.0) atomicAdd(&loss[1], wf * s / (3.0 * n));
+    if (i == 0) atomicAdd(&loss[0], we * de * de / ((double)n * n));
+}
+
+// Adam (Kingma & Ba 2015) with optional decoupled weight decay (AdamW).
+__global__ void k_adam(int n, double* __restrict__ p, const double* __restrict__ g, double* __restrict__ m,
+                       double* __restrict__ v, double lr, double b1, double b2, double eps, double wd,
+                       double c1, double c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double gi = g[i];
+    const double mi = b1 * m[i] + (1.0 - b1) * gi;
+    const double vi = b2 * v[i] + (1.0 - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const double mh = mi / c1, vh = vi / c2;
+    p[i] = p[i] - lr * (mh / (sqrt(vh) + eps) + wd * p[i]);
+}
+
 template <int MODE>
 int launch_mlp(MlpArgs a, hipStream_t st) {
     typedef typename MT<MODE>::T T;
@@ -480,4 +813,63 @@ extern "C" int pt_mlp_energy_grad(int mode, int n_atoms, const void* G, int ldg,
     }
     set_error("pt_mlp_energy_grad: unknown mode %d", mode);
     return PT_ERR_ARG;
+}
+
+extern "C" size_t pt_mlp_param_count(const pt_mlp_desc* desc) {
+    MlpShape s;
+    if (make_shape(desc, s) != PT_OK) return 0;
+    return s.pb[s.n_hidden] + 1;
+}
+
+extern "C" int pt_mlp_train_grad(int mode, int n_atoms, const void* G, const void* T, int ldg,
+                                 int g_precision, const double* scale, const pt_mlp_desc* desc,
+                                 const void* packed, const double* gy, double tseed, double* grad,
+                                 pt_stream_t stream) {
+    PT_REQUIRE(n_atoms >= 0, "pt_mlp_train_grad: n_atoms < 0");
+    if (n_atoms == 0) return PT_OK;
+    PT_REQUIRE(G && T && scale && desc && packed && gy && grad, "pt_mlp_train_grad: null argument");
+    PT_REQUIRE(g_precision == PT_F64 || g_precision == PT_F32, "pt_mlp_train_grad: bad precision");
+    TrainArgs a{};
+    if (int rc = make_shape(desc, a.s)) return rc;
+    PT_REQUIRE(ldg >= desc->in_dim, "pt_mlp_train_grad: ldg < in_dim");
+    a.n_atoms = n_atoms;
+    a.G = G;
+    a.T = T;
+    a.ldg = ldg;
+    a.g_f64 = g_precision == PT_F64;
+    a.scale = scale;
+    a.scale_add = desc->scale_add;
+    a.packed = (const char*)packed;
+    a.gy = gy;
+    a.tseed = tseed;
+    a.grad = grad;
+    a.nparam = (int)(a.s.pb[a.s.n_hidden] + 1);
+    hipStream_t st = (hipStream_t)stream;
+    switch (mode) {
+    case PT_MLP_F64: return launch_train<PT_MLP_F64>(a, st);
+    case PT_MLP_BF16: return launch_train<PT_MLP_BF16>(a, st);
+    case PT_MLP_BF16X3: return launch_train<PT_MLP_BF16X3>(a, st);
+    }
+    set_error("pt_mlp_train_grad: unknown mode %d", mode);
+    return PT_ERR_ARG;
+}
+
+extern "C" int pt_train_seeds(int n, const double* force, const double* force_ref, const double* energy,
+                              double energy_ref, double w_energy, double w_force, double* delta, double* gy,
+                              double* loss, pt_stream_t stream) {
+    PT_REQUIRE(n > 0 && force && force_ref && energy && delta && gy && loss, "pt_train_seeds: bad arguments");
+    hipLaunchKernelGGL(k_train_seeds, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, force,
+                       force_ref, energy, energy_ref, w_energy, w_force, delta, gy, loss);
+    return launch_check("pt_train_seeds");
+}
+
+extern "C" int pt_adam_step(int n, double* params, const double* grad, double* m, double* v, double lr,
+                            double beta1, double beta2, double eps, double weight_decay, int step,
+                            pt_stream_t stream) {
+    PT_REQUIRE(n >= 0 && params && grad && m && v && step >= 1, "pt_adam_step: bad arguments");
+    if (n == 0) return PT_OK;
+    const double c1 = 1.0 - pow(beta1, step), c2 = 1.0 - pow(beta2, step);
+    hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, params, grad, m, v,
+                       lr, beta1, beta2, eps, weight_decay, c1, c2);
+    return launch_check("pt_adam_step");
 }

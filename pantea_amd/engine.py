@@ -70,7 +70,7 @@ class NeighborList:
         pos = positions.to(torch.float64).contiguous()
         stride = int(pos.shape[1])
         types = types.to(torch.int32).contiguous()
-        if n != self.n:
+        if n != self.n or self.ws is None:
             self.n = n
             self.perm = torch.empty(n, dtype=torch.int32, device=dev)
             self.pos4 = torch.empty((n, 4), dtype=torch.float64, device=dev)
@@ -108,6 +108,17 @@ class NeighborList:
             self.type_offsets.append(self.type_offsets[-1] + int(c))
         self._perm_host = None
         return self
+
+    def release_workspace(self) -> None:
+        """Drop the build workspace and the unused capacity of a list that will not be rebuilt.
+
+        A later build() allocates them again.
+        """
+        self.ws = None
+        self.nbr = self.nbr[: max(self.n_pairs, 1)].clone()
+        if self.rev is not None:
+            self.rev = self.rev[: max(self.n_pairs, 1)].clone()
+        self.cap = 0
 
     @property
     def perm_host(self) -> np.ndarray:

@@ -143,8 +143,36 @@ class DeviceMLP:
         self.packed = torch.zeros(nbytes, dtype=torch.uint8, device=device)
         _lib.check(lib.pt_mlp_pack(ctypes.byref(self.desc), flat.data_ptr(), self.packed.data_ptr(),
                                    _lib.stream_ptr()), "pt_mlp_pack")
-        self.flat = flat
+        self.flat = flat  # FP64 master parameters (W then b per layer)
         self.scale = torch.as_tensor(scale_sub_mul, dtype=torch.float64, device=device).contiguous()
+        self.n_params = int(lib.pt_mlp_param_count(ctypes.byref(self.desc)))
+
+    def repack(self) -> None:
+        """Rebuild the packed copies after ``flat`` changed (e.g. an optimiser step)."""
+        _lib.check(_lib.load().pt_mlp_pack(ctypes.byref(self.desc), self.flat.data_ptr(), self.packed.data_ptr(),
+                                           _lib.stream_ptr()), "pt_mlp_pack")
+
+    def params_dict(self) -> ModelParams:
+        """The master parameters back in the layers_{2i}/{kernel,bias} layout (host)."""
+        flat = self.flat.detach().cpu().numpy()
+        out: ModelParams = {}
+        fan_in = self.in_dim
+        o = 0
+        widths = [w for w, _ in self.model.hidden_layers] + [self.model.output_layer[0]]
+        for i, w in enumerate(widths):
+            W = flat[o:o + fan_in * w].reshape(fan_in, w)
+            o += fan_in * w
+            b = flat[o:o + w]
+            o += w
+            out[f"layers_{2 * i}"] = {"kernel": W.copy(), "bias": b.copy()}
+            fan_in = w
+        return out
+
+    def train_grad(self, mode: int, n_atoms: int, G, T, ldg: int, g_precision: int, gy, tseed: float, grad) -> None:
+        _lib.check(_lib.load().pt_mlp_train_grad(
+            mode, n_atoms, G.data_ptr(), T.data_ptr(), ldg, g_precision, self.scale.data_ptr(),
+            ctypes.byref(self.desc), self.packed.data_ptr(), gy.data_ptr(), float(tseed), grad.data_ptr(),
+            _lib.stream_ptr()), "pt_mlp_train_grad")
 
     def energy_grad(self, mode: int, n_atoms: int, G, ldg: int, g_precision: int, e_atom, dEdG) -> None:
         _lib.check(_lib.load().pt_mlp_energy_grad(

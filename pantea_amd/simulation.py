@@ -93,7 +93,7 @@ class _DeviceMD:
         self.ref_pos = nl.pos4.clone()
         self.force = torch.zeros((n, 3), dtype=torch.float64, device=self.dev)
         self.force_new = torch.zeros_like(self.force)
-        self.disp = torch.zeros(1, dtype=torch.float64, device=self.dev)
+        self.disp = torch.zeros(257, dtype=torch.float64, device=self.dev)
         self.n_builds = 1
         self.energy = None
         self.timers = None  # {phase: [(start_event, end_event), ...]} when profiling
@@ -138,7 +138,7 @@ class _DeviceMD:
         _lib.check(lib.pt_max_displacement(self.n, self.nl.pos4.data_ptr(), self.ref_pos.data_ptr(),
                                            ctypes.cast(box, ctypes.c_void_p) if box is not None else None,
                                            self.disp.data_ptr(), _lib.stream_ptr()), "pt_max_displacement")
-        return float(self.disp.item()) > 0.5 * self.skin
+        return float(self.disp[0].item()) > 0.5 * self.skin
 
     # ---- forces --------------------------------------------------------------
     def _blocks(self):

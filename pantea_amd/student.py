@@ -1,0 +1,276 @@
+"""Student NNP trained on the GPU from teacher trajectories.
+
+Pipeline:
+  1. ``teacher_snapshots``: teacher MD (Berendsen, 300 K) on water boxes of
+     uneven size at 1 g/cm^3; every ``every`` steps the configuration is
+     labelled with FP64 teacher energies and forces (F = -grad E).
+  2. ``StudentTrainer``: per snapshot the student's neighbour list and
+     descriptors are built once (positions are fixed); every optimisation
+     step then runs, per structure,
+        MLP energy + dE/dG          pt_mlp_energy_grad
+        E = sum_i E_i               pt_reduce_sum_f64
+        F = -J^T dE/dG              pt_acsf_adjoint + pt_force_gather
+        delta = dL/dF, dL/dE        pt_train_seeds
+        T = J delta                 pt_acsf_tangent (forward mode)
+        dL/dtheta accumulation      pt_mlp_train_grad (forward-over-reverse)
+     and after each batch one Adam step (pt_adam_step) and a repack.
+Loss per structure: w_E ((E - E*)/N)^2 + w_F / (3N) sum |F - F*|^2,
+averaged over the batch.  The reference's own trainer
+(potentials/nnp/gradient_descent.py:157-233) is broken and never run by its
+tests; its loss (energy MSE per atom + beta^2 force MSE) and optax Adam
+defaults (settings.py:116-121) are followed where they apply.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from .atoms import Structure
+from .engine import NeighborList, Reducer, acsf_adjoint, acsf_forward, acsf_tangent, force_gather, gather_rows
+from .potential import PRECISION_MODES, NeuralNetworkPotential
+from .units import units
+
+
+@dataclass
+class Snapshot:
+    """One labelled configuration, resident on the device in internal order."""
+
+    n: int
+    nl: NeighborList
+    blocks: List[Tuple[str, int, int]]   # (element, begin, count) in internal order
+    G: object                            # (n, ld) student descriptors
+    F_ref: object                        # (n, 3) FP64 teacher forces, internal order
+    E_ref: float
+    emap_key: tuple = ()                 # element map (element, type) pairs
+
+
+def teacher_snapshots(teacher: NeuralNetworkPotential, n_structures: int = 64, n_snapshots: int = 2000,
+                      every: int = 10, temperature: float = 300.0, min_molecules: int = 32,
+                      max_molecules: int = 512, seed: int = 2023, md_precision: str = "mixed",
+                      label_precision: str = "float64", tau_fs: float = 25.0, log=None):
+    """Yield (structure, E*, F*) from teacher MD; F* = -grad E (input order, device FP64)."""
+    from . import water
+    from .simulation import BrendsenThermostat, MDSimulator, System
+
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(min_molecules, max_molecules + 1, size=n_structures)
+    per = int(math.ceil(n_snapshots / n_structures))
+    dt = 0.5 * units.FROM_FEMTO_SECOND
+    count = 0
+    for k, nm in enumerate(sizes):
+        if count >= n_snapshots:
+            break
+        L = water.box_for_density(int(nm))
+        s = Structure.from_dict(water.water_box(int(nm), L, seed=seed + 17 * (k + 1)))
+        sysm = System.from_structure(s, teacher, temperature=temperature, seed=seed + k, force_mode="full",
+                                     precision=md_precision, skin=1.0)
+        md = MDSimulator(dt, thermostat=BrendsenThermostat(temperature, tau_fs * units.FROM_FEMTO_SECOND),
+                         integrator="physical")
+        for _ in range(per):
+            if count >= n_snapshots:
+                break
+            md.run(sysm, every)
+            snap = s.copy()
+            snap.positions = sysm.positions.clone()
+            e, f = teacher.energy_and_forces(snap, mode="full", precision=label_precision)
+            count += 1
+            yield snap, float(e), f.clone()
+        if log:
+            log(f"teacher structure {k}: {3 * int(nm)} atoms, {count} snapshots")
+
+
+class StudentTrainer:
+    """Adam training of a NeuralNetworkPotential's MLPs with the GPU kernels."""
+
+    def __init__(self, student: NeuralNetworkPotential, precision: str = "mixed", w_energy: float = 1.0,
+                 w_force: float = 10.0, lr: float = 1e-3, betas=(0.9, 0.999), eps: float = 1e-8,
+                 weight_decay: float = 0.0, seed: int = 0):
+        import torch
+
+        self.pot = student
+        self.dev = _lib.require_device()
+        self.precision = precision
+        self.g_prec, self.mlp_mode = PRECISION_MODES[precision]
+        self.fdt = torch.float64 if self.g_prec == _lib.PT_F64 else torch.float32
+        self.w_energy, self.w_force = float(w_energy), float(w_force)
+        self.lr, self.betas, self.eps, self.wd = float(lr), betas, float(eps), float(weight_decay)
+        self.rng = np.random.default_rng(seed)
+        self.snapshots: List[Snapshot] = []
+        self.rc = student.r_cutoff
+        self.ld = max(ap.model_input_size for ap in student.atomic_potentials.values())
+        self.step_count = 0
+        self.mlps = None
+        self.tables = {}
+        self._raw: List[Tuple[Structure, float, object]] = []
+
+    # ---- data ------------------------------------------------------------------
+    def add(self, structure: Structure, energy: float, forces) -> None:
+        """Queue a labelled structure (forces in input order, device FP64)."""
+        self._raw.append((structure, float(energy), forces))
+
+    def prepare(self, fit_scaler: bool = True) -> None:
+        """Build neighbour lists and descriptors of all queued structures (and fit the scaler)."""
+        import torch
+
+        pot = self.pot
+        for s, e, f in self._raw:
+            emap = s.element_map
+            nl = NeighborList(self.rc, emap.n_types).build(s.positions, s.atom_types, s.box_lengths,
+                                                           s.type_counts(), with_rev=True)
+            blocks = []
+            for el in pot.elements:
+                t = emap.element_to_atom_type.get(el)
+                if t is None:
+                    continue
+                b, cnt = nl.type_block(t)
+                if cnt:
+                    blocks.append((el, b, cnt))
+            G = torch.zeros((s.natoms, self.ld), dtype=self.fdt, device=self.dev)
+            for el, b, cnt in blocks:
+                tab = pot.atomic_potentials[el].descriptor.device_table(emap, self.dev)
+                self.tables[(el, tuple(sorted(emap.element_to_atom_type.items())))] = tab
+                acsf_forward(nl, tab, self.g_prec, cnt, center_begin=b, out=G[b:], ld=self.ld)
+            F_int = gather_rows(nl.perm, f.to(torch.float64).contiguous())
+            nl.release_workspace()
+            self.snapshots.append(Snapshot(s.natoms, nl, blocks, G, F_int, e,
+                                           tuple(sorted(emap.element_to_atom_type.items()))))
+        self._raw = []
+        if fit_scaler:
+            from .descriptors import DescriptorScaler
+
+            for el in pot.elements:
+                params = None
+                for sn in self.snapshots:
+                    for e2, b, cnt in sn.blocks:
+                        if e2 != el:
+                            continue
+                        n_sf = pot.atomic_potentials[el].model_input_size
+                        x = sn.G[b:b + cnt, :n_sf].double().cpu().numpy()
+                        params = DescriptorScaler.fit(x) if params is None else DescriptorScaler.partial_fit(params, x)
+                pot.scalers_params[el] = params
+            pot._mlp_key = None
+        self._init_optimizer()
+
+    def _init_optimizer(self) -> None:
+        import torch
+
+        self.mlps = self.pot._device_mlps(self.dev)
+        self.grad = {e: torch.zeros(m.n_params, dtype=torch.float64, device=self.dev) for e, m in self.mlps.items()}
+        self.m1 = {e: torch.zeros_like(g) for e, g in self.grad.items()}
+        self.m2 = {e: torch.zeros_like(g) for e, g in self.grad.items()}
+        nmax = max(sn.n for sn in self.snapshots)
+        self.e_atom = torch.zeros(nmax, dtype=torch.float64, device=self.dev)
+        self.dEdG = torch.zeros((nmax, self.ld), dtype=self.fdt, device=self.dev)
+        self.T = torch.zeros((nmax, self.ld), dtype=self.fdt, device=self.dev)
+        self.f_self = torch.zeros((nmax, 3), dtype=torch.float64, device=self.dev)
+        self.F = torch.zeros((nmax, 3), dtype=torch.float64, device=self.dev)
+        self.delta = torch.zeros((nmax, 3), dtype=torch.float64, device=self.dev)
+        self.gy = torch.zeros(nmax, dtype=torch.float64, device=self.dev)
+        self.loss = torch.zeros(2, dtype=torch.float64, device=self.dev)
+        self.reducer = Reducer(nmax, self.dev)
+        emax = max(sn.nl.n_pairs for sn in self.snapshots)
+        self.edge = torch.zeros(3 * max(emax, 1), dtype=self.fdt, device=self.dev)
+
+    # ---- one structure ------------------------------------------------------------
+    def _table(self, sn: Snapshot, el: str):
+        return self.tables[(el, sn.emap_key)]
+
+    def _energy_forces(self, sn: Snapshot):
+        """Student E (device scalar) and F (internal order) of a snapshot."""
+        n = sn.n
+        for el, b, cnt in sn.blocks:
+            self.mlps[el].energy_grad(self.mlp_mode, cnt, sn.G[b:], self.ld, self.g_prec, self.e_atom[b:],
+                                      self.dEdG[b:])
+        E = self.reducer.sum(self.e_atom[:n])
+        for el, b, cnt in sn.blocks:
+            acsf_adjoint(sn.nl, self._table(sn, el), self.g_prec, cnt, self.dEdG[b:], self.ld, center_begin=b,
+                         f_self=self.f_self[b:], edge_grad=self.edge)
+        force_gather(sn.nl, self.edge, self.g_prec, out=self.F[:n])
+        return E, self.F[:n]
+
+    def _accumulate(self, sn: Snapshot, scale: float) -> None:
+        n = sn.n
+        E, F = self._energy_forces(sn)
+        _lib.check(_lib.load().pt_train_seeds(n, F.data_ptr(), sn.F_ref.data_ptr(), E.data_ptr(), sn.E_ref,
+                                              self.w_energy * scale, self.w_force * scale, self.delta.data_ptr(),
+                                              self.gy.data_ptr(), self.loss.data_ptr(), _lib.stream_ptr()),
+                   "pt_train_seeds")
+        for el, b, cnt in sn.blocks:
+            acsf_tangent(sn.nl, self._table(sn, el), self.g_prec, cnt, self.delta[:n], center_only=False,
+                         center_begin=b, out=self.T[b:], ld=self.ld)
+        for el, b, cnt in sn.blocks:
+            self.mlps[el].train_grad(self.mlp_mode, cnt, sn.G[b:], self.T[b:], self.ld, self.g_prec, self.gy[b:],
+                                     -1.0, self.grad[el])
+
+    def gradient(self, indices: Sequence[int]):
+        """Loss (energy term, force term) and its gradient over ``indices`` (batch mean)."""
+        for g in self.grad.values():
+            g.zero_()
+        self.loss.zero_()
+        scale = 1.0 / len(indices)
+        for k in indices:
+            self._accumulate(self.snapshots[k], scale)
+        return self.loss.clone(), {e: g.clone() for e, g in self.grad.items()}
+
+    def step(self, indices: Sequence[int]):
+        loss, _ = self.gradient(indices)
+        self.step_count += 1
+        b1, b2 = self.betas
+        for el, mlp in self.mlps.items():
+            _lib.check(_lib.load().pt_adam_step(mlp.n_params, mlp.flat.data_ptr(), self.grad[el].data_ptr(),
+                                                self.m1[el].data_ptr(), self.m2[el].data_ptr(), self.lr, b1, b2,
+                                                self.eps, self.wd, self.step_count, _lib.stream_ptr()),
+                       "pt_adam_step")
+            mlp.repack()
+        return loss
+
+    def train(self, epochs: int, batch: int = 32, train_idx: Optional[Sequence[int]] = None, log=None):
+        import torch
+
+        idx = np.arange(len(self.snapshots)) if train_idx is None else np.asarray(train_idx)
+        history = []
+        t0 = time.perf_counter()
+        for ep in range(epochs):
+            perm = self.rng.permutation(idx)
+            tot = torch.zeros(2, dtype=torch.float64, device=self.dev)
+            nb = 0
+            for s0 in range(0, len(perm), batch):
+                tot += self.step(perm[s0:s0 + batch])
+                nb += 1
+            l = (tot / max(nb, 1)).cpu().numpy()
+            history.append((ep, float(l[0]), float(l[1])))
+            if log:
+                log(f"epoch {ep}: loss_E {l[0]:.4e} loss_F {l[1]:.4e} ({time.perf_counter() - t0:.1f} s)")
+        self.sync_params()
+        return history
+
+    def sync_params(self) -> None:
+        """Copy the device master parameters back into the potential's params dicts."""
+        self.pot.models_params = {e: m.params_dict() for e, m in self.mlps.items()}
+        self.pot._mlp_key = None
+        self.mlps = self.pot._device_mlps(self.dev)
+        # keep optimiser buffers aligned with the rebuilt device MLPs
+        for e, m in self.mlps.items():
+            if m.n_params != self.grad[e].numel():
+                raise RuntimeError("parameter count changed")
+
+    def evaluate(self, indices: Sequence[int]) -> Dict[str, float]:
+        """Energy RMSE (meV/atom) and force RMSE (meV/Bohr) of the student over ``indices``."""
+        import torch
+
+        se, sf, nat = 0.0, 0.0, 0
+        for k in indices:
+            sn = self.snapshots[k]
+            E, F = self._energy_forces(sn)
+            se += ((float(E) - sn.E_ref) / sn.n) ** 2
+            sf += float(((F - sn.F_ref) ** 2).sum())
+            nat += sn.n
+        mev = units.TO_MILLI_ELECTRON_VOLT
+        return {"energy_rmse_mev_per_atom": math.sqrt(se / max(len(indices), 1)) * mev,
+                "force_rmse_mev_per_bohr": math.sqrt(sf / max(3 * nat, 1)) * mev}

@@ -84,6 +84,19 @@ def algorithmic_ops(n_nb, n_trip):
     return fwd, 2.5 * fwd
 
 
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of ``kernel`` (mean over its launch shapes) from the committed PMC summary."""
+    path = ROOT / "profiles" / "hbm_traffic_water98k_md.md"
+    if not path.exists():
+        return None
+    vals = []
+    for line in path.read_text().splitlines():
+        if line.startswith(f"| `{kernel}` |"):
+            cols = [c.strip() for c in line.strip("|").split("|")]
+            vals.append((float(cols[5]) + float(cols[6])) * 1e6)
+    return float(np.mean(vals)) if vals else None
+
+
 def cpu_baseline(pot, s, budget_s=12.0):
     """FP64 oracle (C, OpenMP) timed on a sample of centres of the same structure.
 
@@ -162,6 +175,8 @@ def main() -> int:
     ap.add_argument("--box", type=float, default=187.7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the run, launch 3 plain FP64 copy kernels of MIB MiB each (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -212,21 +227,29 @@ def main() -> int:
     n_nb, n_trip = neighbour_stats(s, pot.r_cutoff)
     fwd_ops, adj_ops = algorithmic_ops(n_nb, n_trip)
     dom = "acsf_adjoint" if phases["acsf_adjoint"][0] >= phases["acsf_forward"][0] else "acsf_forward"
-    dom_ms = phases[dom][0]
-    dom_ops = (adj_ops if dom == "acsf_adjoint" else fwd_ops) * n
-    achieved = dom_ops / (dom_ms * 1e-3) / 1e12 if dom_ms > 0 else 0.0
+    mode = 1 if dom == "acsf_adjoint" else 0
+    # one step launches the kernel twice (O centres, then H centres): per launch = half the pair
+    launch_ms = phases[dom][0] / 2.0
+    launch_ops = (adj_ops if mode else fwd_ops) * n / 2.0
+    achieved = launch_ops / (launch_ms * 1e-3) / 1e12 if launch_ms > 0 else 0.0
+    kname = f"k_acsf<float, {mode}>"
     roofline = {
         "bound": "mfma",
         "achieved": achieved,
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32_TFLOPS,
-        "traffic": None,
-        "kernel": f"k_acsf<float,{'ADJ' if dom == 'acsf_adjoint' else 'FWD'}> (O and H launches of one step)",
-        "note": "compute-bound on the vector ALU in FP32: algorithmic ops (transcendental = 1, FMA = 2) "
-                f"per atom {('2.5 x ' if dom == 'acsf_adjoint' else '')}(63 n_nb + 86 n_trip), "
-                f"n_nb={n_nb:.1f}, n_trip={n_trip:.0f}, measured over {phases[dom][1]} launches with HIP events; "
-                "peak = published MI355X FP32",
+        "traffic": profiled_traffic(kname),
+        "kernel": kname + (" (descriptor adjoint)" if mode else " (descriptor forward)"),
+        "avg_launch_ms": launch_ms,
+        "algorithmic_ops_per_launch": launch_ops,
+        "note": "vector-ALU bound FP32 work (no MFMA in this kernel; 'mfma' is the compute roofline): "
+                f"algorithmic ops per atom {('2.5 x ' if mode else '')}(63 n_nb + 86 n_trip) "
+                "(transcendental = 1, FMA = 2), measured "
+                f"n_nb={n_nb:.1f}, n_trip={n_trip:.0f}; a launch covers the O or the H centres (n/2 on average); "
+                f"average launch time from HIP events over {phases[dom][1]} launch pairs on the library's stream; "
+                "peak = published MI355X FP32; traffic = HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE "
+                "profiles, copy-kernel calibrated",
     }
 
     cpu = None
@@ -236,6 +259,16 @@ def main() -> int:
             _log(f"[bench] cpu baseline: {cpu['value']:.4g} atom-steps/s on {cpu['cores']} threads")
         except Exception as exc:  # the baseline is a report, not the measurement
             _log(f"[bench] cpu baseline failed: {exc!r}")
+
+    if args.copy_probe > 0:
+        # y = 1.0 * x streams exactly 2 * MIB MiB through HBM per launch
+        src = torch.rand(args.copy_probe * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        _sync()
+        _log(f"[bench] copy probe: 3 x {args.copy_probe} MiB read + {args.copy_probe} MiB write")
+        del src, dst
 
     out = {
         "metric": "atom-steps/s, 98k-atom water NNP-MD: mixed+surrogate vs FP64; force RMSE",

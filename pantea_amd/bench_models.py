@@ -65,3 +65,21 @@ def water_nnp(structure, hidden: Sequence[int] = (25, 25), seed: int = 2023, pre
     pot = make_nnp(student_acsf if student else teacher_acsf, hidden, seed, precision)
     pot.fit_scaler([structure])
     return pot
+
+
+def water_teacher(precision: str = "mixed", molecules: int = 32768, box: float = 187.7, seed: int = 2023):
+    """The 98,304-atom water structure and its teacher (MLP 52-64-64-1, scaler fitted on it)."""
+    from . import water
+    from .atoms import Structure
+
+    s = Structure.from_dict(water.water_box(molecules, box, seed=seed))
+    return s, water_nnp(s, hidden=(64, 64), seed=seed, precision=precision)
+
+
+def water_student(directory=None, seed: int = 7, precision: str = "mixed") -> NeuralNetworkPotential:
+    """Student NNP (34 ACSF, MLP 34-16-16-1); loads scaler and weights from ``directory`` when given."""
+    pot = make_nnp(student_acsf, (16, 16), seed, precision)
+    if directory is not None:
+        pot.directory = directory
+        pot.load()
+    return pot

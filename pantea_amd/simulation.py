@@ -25,7 +25,6 @@ device every step); the descriptor kernels re-apply the exact mask
 from __future__ import annotations
 
 import ctypes
-from dataclasses import dataclass
 from pathlib import Path
 from typing import Optional
 
@@ -48,39 +47,112 @@ class BrendsenThermostat:
         self.time_constant = float(time_constant)
 
 
-class _DeviceMD:
-    """Device-resident MD state of one System (internal atom order)."""
+class _ForceField:
+    """Descriptor/MLP buffers and launch sequence of one potential at one precision."""
 
-    def __init__(self, system: "System", skin: float):
+    def __init__(self, pot: NeuralNetworkPotential, precision: str, emap, n: int, dev):
+        import torch
+
+        self.pot = pot
+        self.precision = precision
+        self.g_prec, self.mlp_mode = PRECISION_MODES[precision]
+        self.emap = emap
+        self.n = n
+        self.dev = dev
+        self.rc = pot.r_cutoff
+        self.ld = max(ap.model_input_size for ap in pot.atomic_potentials.values())
+        fdt = torch.float64 if self.g_prec == _lib.PT_F64 else torch.float32
+        self.G = torch.zeros((n, self.ld), dtype=fdt, device=dev)
+        self.dEdG = torch.zeros((n, self.ld), dtype=fdt, device=dev)
+        self.e_atom = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.f_self = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+        self.edge = None
+        self.reducer = Reducer(n, dev)
+        self.tables = {e: pot.atomic_potentials[e].descriptor.device_table(emap, dev)
+                       for e in pot.elements if e in emap.element_to_atom_type}
+        self.covered = all(e in pot.atomic_potentials for e in emap.unique_elements)
+        self.mlps = pot._device_mlps(dev)
+        self.energy = None
+
+    def refresh(self) -> None:
+        """Pick up changed model parameters of the potential."""
+        self.mlps = self.pot._device_mlps(self.dev)
+
+    def _blocks(self, nl: NeighborList):
+        for e in self.pot.elements:
+            t = self.emap.element_to_atom_type.get(e)
+            if t is None:
+                continue
+            b, cnt = nl.type_block(t)
+            if cnt > 0:
+                yield e, b, cnt
+
+    def compute(self, nl: NeighborList, out, force_mode: str, with_energy: bool = False, mark=None):
+        import torch
+
+        mark = mark or (lambda phase, start: None)
+        blocks = list(self._blocks(nl))
+        mark("acsf_forward", True)
+        for e, b, cnt in blocks:
+            acsf_forward(nl, self.tables[e], self.g_prec, cnt, center_begin=b, out=self.G[b:], ld=self.ld)
+        mark("acsf_forward", False)
+        mark("mlp", True)
+        for e, b, cnt in blocks:
+            self.mlps[e].energy_grad(self.mlp_mode, cnt, self.G[b:], self.ld, self.g_prec, self.e_atom[b:],
+                                     self.dEdG[b:])
+        mark("mlp", False)
+        if with_energy:
+            self.energy = self.reducer.sum(self.e_atom)
+        full = force_mode == "full"
+        if full:
+            need = nl.n_pairs * 3
+            edt = torch.float64 if self.g_prec == _lib.PT_F64 else torch.float32
+            if self.edge is None or self.edge.numel() < need:
+                self.edge = torch.zeros(int(need * 1.2) + 3, dtype=edt, device=self.dev)
+            if not self.covered:
+                self.edge.zero_()
+        target = self.f_self if full else out
+        if not self.covered:
+            target.zero_()
+        mark("acsf_adjoint", True)
+        for e, b, cnt in blocks:
+            acsf_adjoint(nl, self.tables[e], self.g_prec, cnt, self.dEdG[b:], self.ld, center_begin=b,
+                         f_self=target[b:], edge_grad=self.edge if full else None)
+        mark("acsf_adjoint", False)
+        if full:
+            mark("force_gather", True)
+            force_gather(nl, self.edge, self.g_prec, out=out)
+            mark("force_gather", False)
+        return out
+
+
+class _DeviceMD:
+    """Device-resident MD state of one System (internal atom order).
+
+    ``extra`` potentials (e.g. a surrogate and its judge) share the state and
+    the neighbour list, whose cutoff is the largest of all plus the skin.
+    """
+
+    def __init__(self, system: "System", skin: float, extra=()):
         import torch
 
         pot = system.potential
         s = system.structure
-        self.pot = pot
         self.dev = s.device
         self.n = s.natoms
         self.emap = s.element_map
         self.box = s.box_lengths
         self.skin = float(skin)
-        self.g_prec, self.mlp_mode = PRECISION_MODES[system.precision]
-        self.rc = pot.r_cutoff
+        n = self.n
+        self.fields = {(id(pot), system.precision): _ForceField(pot, system.precision, self.emap, n, self.dev)}
+        for p, prec in extra:
+            self.fields.setdefault((id(p), prec), _ForceField(p, prec, self.emap, n, self.dev))
+        self.ff = self.fields[(id(pot), system.precision)]
+        self.rc = max(f.rc for f in self.fields.values())
         self.nls = [NeighborList(self.rc + self.skin, self.emap.n_types),
                     NeighborList(self.rc + self.skin, self.emap.n_types)]
         self.cur = 0
         self.type_counts = s.type_counts()
-        self.ld = max(ap.model_input_size for ap in pot.atomic_potentials.values())
-        fdt = torch.float64 if self.g_prec == _lib.PT_F64 else torch.float32
-        n = self.n
-        self.G = torch.zeros((n, self.ld), dtype=fdt, device=self.dev)
-        self.dEdG = torch.zeros((n, self.ld), dtype=fdt, device=self.dev)
-        self.e_atom = torch.zeros(n, dtype=torch.float64, device=self.dev)
-        self.f_self = torch.zeros((n, 3), dtype=torch.float64, device=self.dev)
-        self.edge = None
-        self.reducer = Reducer(n, self.dev)
-        self.tables = {e: pot.atomic_potentials[e].descriptor.device_table(self.emap, self.dev)
-                       for e in pot.elements if e in self.emap.element_to_atom_type}
-        self.covered = all(e in pot.atomic_potentials for e in self.emap.unique_elements)
-        self.mlps = pot._device_mlps(self.dev)
         # state in input order -> internal order of the first build
         nl = self.nls[0].build(s.positions, s.atom_types, self.box, self.type_counts, with_rev=True)
         perm = nl.perm
@@ -94,9 +166,44 @@ class _DeviceMD:
         self.force = torch.zeros((n, 3), dtype=torch.float64, device=self.dev)
         self.force_new = torch.zeros_like(self.force)
         self.disp = torch.zeros(257, dtype=torch.float64, device=self.dev)
+        self.reducer = Reducer(n, self.dev)
         self.n_builds = 1
-        self.energy = None
         self.timers = None  # {phase: [(start_event, end_event), ...]} when profiling
+
+    def add_field(self, pot, precision: str) -> _ForceField:
+        """Evaluate ``pot`` at ``precision`` on this state too (grows the list cutoff if needed)."""
+        key = (id(pot), precision)
+        if key not in self.fields:
+            self.fields[key] = _ForceField(pot, precision, self.emap, self.n, self.dev)
+            rc = max(f.rc for f in self.fields.values())
+            if rc > self.rc:
+                self.rc = rc
+                pos = self.nl.pos4.clone()
+                self.nls = [NeighborList(rc + self.skin, self.emap.n_types),
+                            NeighborList(rc + self.skin, self.emap.n_types)]
+                self.cur = 0
+                self._rebuild(pos)
+        return self.fields[key]
+
+    # the driving potential's buffers (kept for callers that read them)
+    @property
+    def pot(self):
+        return self.ff.pot
+
+    @property
+    def energy(self):
+        return self.ff.energy
+
+    @property
+    def e_atom(self):
+        return self.ff.e_atom
+
+    def field(self, pot, precision: str) -> _ForceField:
+        return self.fields[(id(pot), precision)]
+
+    def use(self, field: _ForceField) -> None:
+        """Make ``field`` the driving potential of step()."""
+        self.ff = field
 
     def _mark(self, phase: str, start: bool) -> None:
         if self.timers is None:
@@ -115,10 +222,12 @@ class _DeviceMD:
         return self.nls[self.cur]
 
     # ---- neighbour maintenance ---------------------------------------------
-    def _rebuild(self) -> None:
+    def _rebuild(self, positions=None) -> None:
+        """New list (and internal order) from ``positions`` (current order; default: current positions)."""
         old = self.nls[self.cur]
         new = self.nls[1 - self.cur]
-        new.build(old.pos4, self.types, self.box, self.type_counts, with_rev=True)
+        new.build(old.pos4 if positions is None else positions, self.types, self.box, self.type_counts,
+                  with_rev=True)
         p = new.perm
         self.orig = gather_rows(p, self.orig.reshape(-1, 1)).reshape(-1).contiguous()
         self.types = gather_rows(p, self.types.reshape(-1, 1)).reshape(-1).contiguous()
@@ -140,54 +249,24 @@ class _DeviceMD:
                                            self.disp.data_ptr(), _lib.stream_ptr()), "pt_max_displacement")
         return float(self.disp[0].item()) > 0.5 * self.skin
 
+    # ---- state snapshots (rewind) -------------------------------------------------
+    def save_state(self) -> dict:
+        return {"pos": self.nl.pos4.clone(), "vel": self.vel.clone(), "force": self.force.clone(),
+                "orig": self.orig.clone(), "types": self.types.clone(), "mass": self.mass.clone(),
+                "inv_mass": self.inv_mass.clone()}
+
+    def restore_state(self, st: dict) -> None:
+        self.orig = st["orig"].clone()
+        self.types = st["types"].clone()
+        self.vel = st["vel"].clone()
+        self.force = st["force"].clone()
+        self.mass = st["mass"].clone()
+        self.inv_mass = st["inv_mass"].clone()
+        self._rebuild(st["pos"])
+
     # ---- forces --------------------------------------------------------------
-    def _blocks(self):
-        nl = self.nl
-        for e in self.pot.elements:
-            t = self.emap.element_to_atom_type.get(e)
-            if t is None:
-                continue
-            b, cnt = nl.type_block(t)
-            if cnt > 0:
-                yield e, b, cnt
-
-    def compute(self, out, force_mode: str, with_energy: bool = False):
-        import torch
-
-        nl = self.nl
-        blocks = list(self._blocks())
-        self._mark("acsf_forward", True)
-        for e, b, cnt in blocks:
-            acsf_forward(nl, self.tables[e], self.g_prec, cnt, center_begin=b, out=self.G[b:], ld=self.ld)
-        self._mark("acsf_forward", False)
-        self._mark("mlp", True)
-        for e, b, cnt in blocks:
-            self.mlps[e].energy_grad(self.mlp_mode, cnt, self.G[b:], self.ld, self.g_prec, self.e_atom[b:],
-                                     self.dEdG[b:])
-        self._mark("mlp", False)
-        if with_energy:
-            self.energy = self.reducer.sum(self.e_atom)
-        full = force_mode == "full"
-        if full:
-            need = nl.n_pairs * 3
-            edt = torch.float64 if self.g_prec == _lib.PT_F64 else torch.float32
-            if self.edge is None or self.edge.numel() < need:
-                self.edge = torch.zeros(int(need * 1.2) + 3, dtype=edt, device=self.dev)
-            if not self.covered:
-                self.edge.zero_()
-        target = self.f_self if full else out
-        if not self.covered:
-            target.zero_()
-        self._mark("acsf_adjoint", True)
-        for e, b, cnt in blocks:
-            acsf_adjoint(nl, self.tables[e], self.g_prec, cnt, self.dEdG[b:], self.ld, center_begin=b,
-                         f_self=target[b:], edge_grad=self.edge if full else None)
-        self._mark("acsf_adjoint", False)
-        if full:
-            self._mark("force_gather", True)
-            force_gather(nl, self.edge, self.g_prec, out=out)
-            self._mark("force_gather", False)
-        return out
+    def compute(self, out, force_mode: str, with_energy: bool = False, field: Optional[_ForceField] = None):
+        return (field or self.ff).compute(self.nl, out, force_mode, with_energy, self._mark)
 
     # ---- one step ------------------------------------------------------------
     def step(self, dt: float, integrator: str, force_mode: str, thermostat: Optional[BrendsenThermostat]):
@@ -224,21 +303,41 @@ class _DeviceMD:
         return self.to_input(self.nl.pos4[:, :3].contiguous())
 
 
-@dataclass
 class System:
-    """Structure + potential + velocities + masses (system.py:55-164)."""
+    """Structure + potential + velocities + masses (system.py:55-164).
 
-    potential: NeuralNetworkPotential
-    structure: object
-    velocities: object
-    masses: object
-    force_mode: str = "reference"
-    precision: str = "float64"
-    skin: float = 0.0
+    Positions, velocities and forces live on the device in the MD engine;
+    the properties return them in input atom order.
+    """
 
-    def __post_init__(self) -> None:
+    def __init__(self, potential: NeuralNetworkPotential, structure, velocities, masses,
+                 force_mode: str = "reference", precision: str = "float64", skin: float = 0.0):
+        if force_mode not in ("reference", "full"):
+            raise ValueError(f"unknown force mode '{force_mode}'")
+        self.potential = potential
+        self.structure = structure
+        self._velocities = velocities
+        self.masses = masses
+        self.force_mode = force_mode
+        self.precision = precision
+        self.skin = float(skin)
         self._md: Optional[_DeviceMD] = None
         self.update_forces_from_positions()
+
+    @property
+    def velocities(self):
+        if self._md is not None:
+            self._velocities = self._md.to_input(self._md.vel)
+        return self._velocities
+
+    @velocities.setter
+    def velocities(self, value) -> None:
+        import torch
+
+        value = torch.as_tensor(value, dtype=torch.float64, device=self.masses.device).reshape(-1, 3).contiguous()
+        self._velocities = value
+        if self._md is not None:
+            self._md.vel = gather_rows(self._md.orig, value)
 
     @classmethod
     def from_structure(cls, structure, potential, temperature: float = 300.0, seed: int = 2024,
@@ -263,9 +362,9 @@ class System:
         return torch.as_tensor(v, dtype=torch.float64, device=masses.device)
 
     # ---- device state ------------------------------------------------------------
-    def _engine(self) -> _DeviceMD:
+    def _engine(self, extra=()) -> _DeviceMD:
         if self._md is None:
-            self._md = _DeviceMD(self, self.skin)
+            self._md = _DeviceMD(self, self.skin, extra)
         return self._md
 
     def _sync_from_engine(self) -> None:
@@ -274,7 +373,6 @@ class System:
             return
         self.structure.positions = md.positions().to(self.structure.dtype)
         self.structure.forces = md.to_input(md.force)
-        self.velocities = md.to_input(md.vel)
 
     def update_forces_from_positions(self) -> None:
         md = self._engine()

@@ -53,7 +53,7 @@ class Snapshot:
 def teacher_snapshots(teacher: NeuralNetworkPotential, n_structures: int = 64, n_snapshots: int = 2000,
                       every: int = 10, temperature: float = 300.0, min_molecules: int = 32,
                       max_molecules: int = 512, seed: int = 2023, md_precision: str = "mixed",
-                      label_precision: str = "float64", tau_fs: float = 25.0, log=None):
+                      label_precision: str = "float64", tau_fs: float = 0.5, log=None):
     """Yield (structure, E*, F*) from teacher MD; F* = -grad E (input order, device FP64)."""
     from . import water
     from .simulation import BrendsenThermostat, MDSimulator, System
@@ -274,3 +274,73 @@ class StudentTrainer:
         mev = units.TO_MILLI_ELECTRON_VOLT
         return {"energy_rmse_mev_per_atom": math.sqrt(se / max(len(indices), 1)) * mev,
                 "force_rmse_mev_per_bohr": math.sqrt(sf / max(3 * nat, 1)) * mev}
+
+
+ASSET_DIR = __import__("pathlib").Path(__file__).resolve().parent / "assets" / "student_water"
+
+
+def train_water_student(out_dir=ASSET_DIR, n_snapshots: int = 2000, n_structures: int = 64, every: int = 10,
+                        epochs: int = 200, batch: int = 32, lr: float = 1e-3, w_force: float = 10.0,
+                        seed: int = 2023, log=print) -> dict:
+    """Generate teacher data, train the water student on the GPU and save it to ``out_dir``.
+
+    Teacher: the 98,304-atom benchmark NNP (52 ACSF, 52-64-64-1); data: its own
+    300 K MD on ``n_structures`` water boxes of 32..512 molecules at 1 g/cm^3,
+    ``n_snapshots`` snapshots every ``every`` steps with FP64 labels; student:
+    34 ACSF, 34-16-16-1, mixed precision, Adam, loss E-MSE/atom + w_force F-MSE.
+    """
+    import json
+    from pathlib import Path
+
+    from . import bench_models
+
+    t0 = time.perf_counter()
+    _, teacher = bench_models.water_teacher(precision="mixed", seed=seed)
+    log(f"[student] teacher ready ({time.perf_counter() - t0:.1f} s)")
+    student = bench_models.water_student(seed=seed + 5)
+    tr = StudentTrainer(student, precision="mixed", w_energy=1.0, w_force=w_force, lr=lr, seed=seed)
+    for snap, e, f in teacher_snapshots(teacher, n_structures=n_structures, n_snapshots=n_snapshots,
+                                        every=every, seed=seed, log=log):
+        tr.add(snap, e, f)
+    log(f"[student] {len(tr._raw)} snapshots labelled ({time.perf_counter() - t0:.1f} s)")
+    tr.prepare(fit_scaler=True)
+    n = len(tr.snapshots)
+    perm = np.random.default_rng(seed).permutation(n)
+    n_val = max(1, n // 10)
+    val, train = np.sort(perm[:n_val]), np.sort(perm[n_val:])
+    before = tr.evaluate(val)
+    log(f"[student] untrained: {before}")
+    t1 = time.perf_counter()
+    hist = tr.train(epochs, batch=batch, train_idx=train,
+                    log=lambda m: log(f"[student] {m}"))
+    t_train = time.perf_counter() - t1
+    after_val = tr.evaluate(val)
+    after_train = tr.evaluate(train)
+    log(f"[student] trained: validation {after_val}, training {after_train} ({t_train:.1f} s)")
+    out = Path(out_dir)
+    out.mkdir(parents=True, exist_ok=True)
+    student.save(out)
+    f_rms = math.sqrt(sum(float((sn.F_ref ** 2).sum()) for sn in tr.snapshots) /
+                      (3 * sum(sn.n for sn in tr.snapshots))) * units.TO_MILLI_ELECTRON_VOLT
+    metrics = {"snapshots": n, "validation": int(n_val), "epochs": epochs, "batch": batch, "lr": lr,
+               "w_force": w_force, "adam_steps": tr.step_count, "train_seconds": t_train,
+               "untrained_validation": before, "validation_metrics": after_val, "training_metrics": after_train,
+               "teacher_force_rms_mev_per_bohr": f_rms,
+               "loss_history": [(int(a), float(b), float(c)) for a, b, c in hist[:: max(1, len(hist) // 50)]]}
+    (out / "metrics.json").write_text(json.dumps(metrics, indent=1))
+    return metrics
+
+
+if __name__ == "__main__":
+    import argparse
+
+    ap = argparse.ArgumentParser(description="train the water student NNP on teacher MD data (GPU)")
+    ap.add_argument("--out", default=str(ASSET_DIR))
+    ap.add_argument("--snapshots", type=int, default=2000)
+    ap.add_argument("--structures", type=int, default=64)
+    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    a = ap.parse_args()
+    train_water_student(a.out, a.snapshots, a.structures, epochs=a.epochs, batch=a.batch, lr=a.lr,
+                        log=lambda m: print(m, flush=True))

@@ -1,0 +1,160 @@
+"""Surrogate-driven MD with a judge and fallback.
+
+A student NNP drives velocity-Verlet MD.  Every ``judge_every`` steps the
+teacher NNP (FP64 by default) is evaluated on the current positions and its
+forces are compared with the student's (RMSE over all force components).
+When the RMSE exceeds ``threshold`` the run rewinds to the last state that
+passed and the teacher drives the next ``fallback_steps`` steps; then the
+student takes over again from the state the teacher reached.
+
+Student, judge and fallback potentials share one device state and one
+neighbour list (cutoff = the largest of them + skin), so switching the
+driver costs one force evaluation and no data movement.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from .simulation import BrendsenThermostat, System
+from .units import units
+
+
+@dataclass
+class SurrogateStats:
+    steps: int = 0                 # net trajectory length (steps kept)
+    student_steps: int = 0         # steps driven by the student (kept or discarded)
+    fallback_steps: int = 0        # steps driven by the teacher
+    discarded_steps: int = 0       # student steps thrown away by rewinds
+    judges: int = 0
+    failures: int = 0
+    judge_rmse: list = field(default_factory=list)   # Ha/Bohr, one per judge
+    seconds: float = 0.0
+
+    @property
+    def fallback_rate(self) -> float:
+        """Share of judged windows that failed."""
+        return self.failures / self.judges if self.judges else 0.0
+
+    @property
+    def teacher_share(self) -> float:
+        """Share of the kept trajectory driven by the teacher."""
+        return self.fallback_steps / self.steps if self.steps else 0.0
+
+    def as_dict(self) -> dict:
+        mev = units.TO_MILLI_ELECTRON_VOLT
+        r = np.asarray(self.judge_rmse, dtype=np.float64) * mev
+        return {"steps": self.steps, "student_steps": self.student_steps, "fallback_steps": self.fallback_steps,
+                "discarded_steps": self.discarded_steps, "judges": self.judges, "failures": self.failures,
+                "fallback_rate": self.fallback_rate, "teacher_share": self.teacher_share,
+                "judge_rmse_mev_per_bohr_median": float(np.median(r)) if r.size else None,
+                "judge_rmse_mev_per_bohr_max": float(np.max(r)) if r.size else None,
+                "seconds": self.seconds}
+
+
+class SurrogateMD:
+    """Student-driven MD judged by the teacher (see the module docstring)."""
+
+    def __init__(self, teacher, student, structure, temperature: float = 300.0, seed: int = 2023,
+                 time_step: float = 0.5 * units.FROM_FEMTO_SECOND, integrator: str = "physical",
+                 student_precision: str = "mixed", judge_precision: str = "float64",
+                 fallback_precision: str = "float64", judge_every: int = 20,
+                 threshold: float = 5.0 / units.TO_MILLI_ELECTRON_VOLT, fallback_steps: int = 200,
+                 skin: float = 1.0, thermostat: Optional[BrendsenThermostat] = None):
+        import torch
+
+        if integrator not in ("reference", "physical"):
+            raise ValueError(f"unknown integrator '{integrator}'")
+        if judge_every < 1 or fallback_steps < 1:
+            raise ValueError("judge_every and fallback_steps must be >= 1")
+        self.system = System.from_structure(structure, student, temperature=temperature, seed=seed,
+                                            force_mode="full", precision=student_precision, skin=skin)
+        md = self.system._engine()
+        self.md = md
+        self.f_student = md.ff
+        self.f_judge = md.add_field(teacher, judge_precision)
+        self.f_fallback = md.add_field(teacher, fallback_precision)
+        # the list may have grown: forces of the driver at the current positions
+        md.compute(md.force, "full")
+        self.dt = float(time_step)
+        self.integrator = integrator
+        self.thermostat = thermostat
+        self.judge_every = int(judge_every)
+        self.threshold = float(threshold)
+        self.fallback_len = int(fallback_steps)
+        self.f_judge_out = torch.zeros_like(md.force)
+        self.stats = SurrogateStats()
+        self.mode = "student"
+        self.fallback_left = 0
+        self.good = md.save_state()
+        self.good_step = 0
+
+    # ---- judge -------------------------------------------------------------------
+    def judge(self) -> float:
+        """Force RMSE (Ha/Bohr) of the driving student against the judge at the current positions."""
+        import torch
+
+        md = self.md
+        md.compute(self.f_judge_out, "full", field=self.f_judge)
+        diff = md.force - self.f_judge_out
+        return math.sqrt(float((diff * diff).mean()))
+
+    # ---- driving -----------------------------------------------------------------
+    def _to_student(self) -> None:
+        md = self.md
+        md.use(self.f_student)
+        md.compute(md.force, "full")
+        self.good = md.save_state()
+        self.good_step = self.stats.steps
+        self.mode = "student"
+
+    def _to_fallback(self) -> None:
+        md = self.md
+        self.stats.discarded_steps += self.stats.steps - self.good_step
+        md.restore_state(self.good)
+        self.stats.steps = self.good_step
+        md.use(self.f_fallback)
+        md.compute(md.force, "full")
+        self.mode = "fallback"
+        self.fallback_left = self.fallback_len
+
+    def run(self, n_steps: int) -> SurrogateStats:
+        """Advance the kept trajectory by ``n_steps`` steps."""
+        import torch
+
+        md = self.md
+        st = self.stats
+        target = st.steps + int(n_steps)
+        t0 = time.perf_counter()
+        while st.steps < target:
+            md.step(self.dt, self.integrator, "full", self.thermostat)
+            st.steps += 1
+            if self.mode == "fallback":
+                st.fallback_steps += 1
+                self.fallback_left -= 1
+                if self.fallback_left == 0:
+                    self._to_student()
+                continue
+            st.student_steps += 1
+            if st.steps - self.good_step >= self.judge_every:
+                rmse = self.judge()
+                st.judges += 1
+                st.judge_rmse.append(rmse)
+                if rmse <= self.threshold:
+                    self.good = md.save_state()
+                    self.good_step = st.steps
+                else:
+                    st.failures += 1
+                    self._to_fallback()
+        torch.cuda.synchronize()
+        st.seconds += time.perf_counter() - t0
+        return st
+
+    @property
+    def positions(self):
+        return self.md.positions()

@@ -1,0 +1,220 @@
+"""Velocity Verlet, thermostat and surrogate-driven MD on the GPU against the oracle.
+
+One MD step is checked against the oracle's closed forms
+(molecular_dynamics.py:16-30, thermostat.py:12-22) fed with the oracle's own
+FP64 forces; the long runs check invariants (NVE energy drift, the surrogate
+driver's bookkeeping and that a rewound, teacher-driven run is the teacher's
+trajectory).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle
+from oracle.bridge import oracle_nnp
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg0(hidden=(25, 25), n_mol=128, L=29.56, seed=2023):
+    from pantea_amd import water
+    from pantea_amd.atoms import Structure
+    from pantea_amd.bench_models import water_nnp
+
+    s = Structure.from_dict(water.water_box(n_mol, L, seed=seed))
+    return s, water_nnp(s, hidden=hidden, seed=seed, precision="float64")
+
+
+@pytest.mark.parametrize("integrator,force_mode", [("physical", "full"), ("reference", "reference")])
+@pytest.mark.parametrize("skin", [0.0, 1.0])
+def test_verlet_step_matches_oracle(integrator, force_mode, skin):
+    from pantea_amd.simulation import BrendsenThermostat, MDSimulator, System
+    from pantea_amd.units import units
+
+    s, pot = _cfg0()
+    orc = oracle_nnp(pot, s)
+    sysm = System.from_structure(s, pot, temperature=300.0, seed=11, force_mode=force_mode,
+                                 precision="float64", skin=skin)
+    dt = 0.5 * units.FROM_FEMTO_SECOND
+    md = MDSimulator(dt, integrator=integrator)
+    x0 = sysm.positions.cpu().numpy().copy()
+    v0 = sysm.velocities.cpu().numpy().copy()
+    m = sysm.masses.cpu().numpy().reshape(-1, 1)
+    types = s.atom_types_host
+    box = np.asarray(s.box_lengths)
+    col = 3 if force_mode == "full" else 2
+
+    def acc(x):
+        f = orc.energy_forces(x, types, box)[col]
+        return f / m if integrator == "physical" else f, f
+
+    # each step is checked as a map from the GPU's state before it, so that rounding does not
+    # compound through the trajectory
+    a0, _ = acc(x0)
+    for step in range(3):
+        md.simulate_one_step(sysm)
+        x1 = pyoracle.verlet_positions(x0, v0, a0, dt, box)
+        xg = sysm.positions.cpu().numpy()
+        d = xg - x1
+        d -= box * np.round(d / box)          # a coordinate may wrap on the other side of 0 / L
+        # the reference integrator (a = F, no mass) moves atoms by tens of Bohr per step, so the
+        # forces' 1e-10-level rounding shows up scaled by the step length
+        disp = np.max(np.abs(v0 * dt + 0.5 * a0 * dt * dt))
+        assert np.max(np.abs(d)) < 1e-11 + 1e-9 * disp, (step, np.max(np.abs(d)), disp)
+        a1, f1 = acc(xg)
+        v1 = pyoracle.verlet_velocities(v0, a0, a1, dt)
+        # componentwise relative to the largest component: cancelling components carry the
+        # rounding of their large terms
+        np.testing.assert_allclose(sysm.forces.cpu().numpy(), f1, rtol=0, atol=1e-10 * np.max(np.abs(f1)))
+        np.testing.assert_allclose(sysm.velocities.cpu().numpy(), v1, rtol=0, atol=1e-10 * np.max(np.abs(v1)))
+        x0, v0, a0 = xg.copy(), sysm.velocities.cpu().numpy().copy(), a1
+    assert md.step == 3 and md.elapsed_time == pytest.approx(3 * dt)
+
+
+def test_berendsen_step_matches_oracle():
+    from pantea_amd.simulation import BrendsenThermostat, MDSimulator, System
+    from pantea_amd.units import units
+
+    s, pot = _cfg0()
+    orc = oracle_nnp(pot, s)
+    sysm = System.from_structure(s, pot, temperature=450.0, seed=3, force_mode="full", precision="float64")
+    dt = 0.5 * units.FROM_FEMTO_SECOND
+    tau = 10.0 * dt
+    md = MDSimulator(dt, thermostat=BrendsenThermostat(300.0, tau), integrator="physical")
+    x0 = sysm.positions.cpu().numpy().copy()
+    v0 = sysm.velocities.cpu().numpy().copy()
+    m = sysm.masses.cpu().numpy().reshape(-1, 1)
+    types, box = s.atom_types_host, np.asarray(s.box_lengths)
+    a0 = orc.energy_forces(x0, types, box)[3] / m
+    md.simulate_one_step(sysm)
+    x1 = pyoracle.verlet_positions(x0, v0, a0, dt, box)
+    a1 = orc.energy_forces(x1, types, box)[3] / m
+    v1 = pyoracle.verlet_velocities(v0, a0, a1, dt)
+    lam = pyoracle.berendsen_factor(dt, tau, pyoracle.temperature(v1, m), 300.0)
+    assert lam < 1.0
+    np.testing.assert_allclose(sysm.velocities.cpu().numpy(), lam * v1, rtol=1e-9, atol=1e-14)
+    assert float(sysm.get_temperature()) == pytest.approx(pyoracle.temperature(lam * v1, m), rel=1e-10)
+
+
+def test_maxwell_boltzmann_velocities():
+    """T = 300 K exactly before the COM shift (system.py:83-96), zero COM velocity after it."""
+    from pantea_amd.simulation import System
+
+    s, pot = _cfg0()
+    sysm = System.from_structure(s, pot, temperature=300.0, seed=5, force_mode="full")
+    v = sysm.velocities.cpu().numpy()
+    m = sysm.masses.cpu().numpy().reshape(-1, 1)
+    np.testing.assert_allclose((m * v).sum(0) / m.sum(), 0.0, atol=1e-15)
+    assert float(sysm.get_temperature()) == pytest.approx(300.0, rel=0.02)
+
+
+def test_nve_energy_error_is_second_order_in_dt():
+    """Velocity Verlet with F = -grad E: the total-energy error after 5 fs falls as dt^2.
+
+    The random-weight water NNP with its min-max scaler heats the box from 300 K
+    to ~5e4 K within those 5 fs, which makes this a demanding check of the
+    integrator and of F = -grad E together.
+    """
+    from pantea_amd.simulation import MDSimulator, System
+    from pantea_amd.units import units
+
+    s, pot = _cfg0()
+    errs = []
+    for dt_fs in (0.5, 0.25, 0.125):
+        sysm = System.from_structure(s, pot, temperature=300.0, seed=2023, force_mode="full",
+                                     precision="float64", skin=1.0)
+        md = MDSimulator(dt_fs * units.FROM_FEMTO_SECOND, integrator="physical")
+        e0 = float(sysm.get_total_energy())
+        md.run(sysm, int(round(5.0 / dt_fs)))
+        errs.append(abs(float(sysm.get_total_energy()) - e0))
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    print(f"|dE| at 5 fs for dt 0.5/0.25/0.125 fs: {errs}; ratios {r1:.2f} {r2:.2f}")
+    assert 3.0 < r1 < 5.5 and 3.0 < r2 < 5.5
+
+
+def test_nve_cfg1_1000_steps():
+    """1000 NVE steps (dt 0.5 fs, a = F/m, F = -grad E) on 4096 H2O stay finite; the drift is reported."""
+    from pantea_amd import water
+    from pantea_amd.atoms import Structure
+    from pantea_amd.bench_models import water_nnp
+    from pantea_amd.simulation import MDSimulator, System
+    from pantea_amd.units import units
+
+    s = Structure.from_dict(water.water_box(4096, 93.85, seed=2023))
+    pot = water_nnp(s, hidden=(25, 25), seed=2023, precision="float64")
+    sysm = System.from_structure(s, pot, temperature=300.0, seed=2023, force_mode="full", precision="float64",
+                                 skin=1.0)
+    md = MDSimulator(0.5 * units.FROM_FEMTO_SECOND, integrator="physical")
+    e0 = float(sysm.get_total_energy())
+    n = s.natoms
+    mev = units.TO_MILLI_ELECTRON_VOLT
+    log = []
+    for k in range(10):
+        md.run(sysm, 100)
+        e = float(sysm.get_total_energy())
+        assert math.isfinite(e)
+        log.append(f"{100 * (k + 1)}: dE {(e - e0) / n * mev:.4g} meV/atom T {float(sysm.get_temperature()):.0f} K")
+    print("NVE cfg1: " + "; ".join(log))
+    assert md.step == 1000
+    assert sysm._engine().n_builds > 1
+
+
+def test_rewind_restores_state_exactly():
+    from pantea_amd.simulation import MDSimulator, System
+    from pantea_amd.units import units
+
+    s, pot = _cfg0()
+    sysm = System.from_structure(s, pot, temperature=300.0, seed=2, force_mode="full", precision="float64",
+                                 skin=1.0)
+    md = MDSimulator(0.5 * units.FROM_FEMTO_SECOND, integrator="physical")
+    md.run(sysm, 5)
+    eng = sysm._engine()
+    st = eng.save_state()
+    x_saved = sysm.positions.clone()
+    md.run(sysm, 7)
+    x_a = sysm.positions.clone()
+    eng.restore_state(st)
+    assert float((sysm.positions - x_saved).abs().max()) == 0.0
+    md.run(sysm, 7)
+    assert float((sysm.positions - x_a).abs().max()) < 1e-12
+
+
+def test_surrogate_with_teacher_as_student_never_falls_back():
+    from pantea_amd.simulation import MDSimulator, System
+    from pantea_amd.surrogate import SurrogateMD
+    from pantea_amd.units import units
+
+    s, pot = _cfg0()
+    sur = SurrogateMD(pot, pot, s, seed=4, student_precision="float64", judge_every=5, skin=1.0)
+    st = sur.run(30)
+    assert st.steps == 30 and st.judges == 6 and st.failures == 0 and st.fallback_steps == 0
+    assert max(st.judge_rmse) < 1e-12
+    plain = System.from_structure(s, pot, temperature=300.0, seed=4, force_mode="full", precision="float64",
+                                  skin=1.0)
+    MDSimulator(0.5 * units.FROM_FEMTO_SECOND, integrator="physical").run(plain, 30)
+    assert float((sur.positions - plain.positions).abs().max()) < 1e-10
+
+
+def test_surrogate_failed_judge_rewinds_to_teacher_trajectory():
+    """threshold 0: every student window is rejected, so the kept trajectory is the teacher's own."""
+    from pantea_amd.bench_models import make_nnp, student_acsf
+    from pantea_amd.simulation import MDSimulator, System
+    from pantea_amd.surrogate import SurrogateMD
+    from pantea_amd.units import units
+
+    s, teacher = _cfg0()
+    student = make_nnp(student_acsf, (16, 16), seed=99, precision="mixed")
+    student.fit_scaler([s])
+    sur = SurrogateMD(teacher, student, s, seed=8, judge_every=4, threshold=0.0, fallback_steps=10, skin=1.0)
+    st = sur.run(25)
+    # windows: 4 student steps rejected, 10 teacher steps, ... -> kept steps all teacher-driven
+    assert st.steps == 25
+    assert st.fallback_steps == 25
+    assert st.failures == st.judges == 3
+    assert st.discarded_steps == 12
+    plain = System.from_structure(s, teacher, temperature=300.0, seed=8, force_mode="full", precision="float64",
+                                  skin=1.0)
+    MDSimulator(0.5 * units.FROM_FEMTO_SECOND, integrator="physical").run(plain, 25)
+    assert float((sur.positions - plain.positions).abs().max()) < 1e-9

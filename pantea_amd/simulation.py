@@ -55,6 +55,7 @@ class _ForceField:
 
         self.pot = pot
         self.precision = precision
+        self.label = precision  # timer prefix; owners may rename it
         self.g_prec, self.mlp_mode = PRECISION_MODES[precision]
         self.emap = emap
         self.n = n
@@ -266,7 +267,9 @@ class _DeviceMD:
 
     # ---- forces --------------------------------------------------------------
     def compute(self, out, force_mode: str, with_energy: bool = False, field: Optional[_ForceField] = None):
-        return (field or self.ff).compute(self.nl, out, force_mode, with_energy, self._mark)
+        f = field or self.ff
+        tag = "" if len(self.fields) == 1 else f"{f.label}/"
+        return f.compute(self.nl, out, force_mode, with_energy, lambda ph, st: self._mark(tag + ph, st))
 
     # ---- one step ------------------------------------------------------------
     def step(self, dt: float, integrator: str, force_mode: str, thermostat: Optional[BrendsenThermostat]):

@@ -79,6 +79,9 @@ class SurrogateMD:
         self.f_student = md.ff
         self.f_judge = md.add_field(teacher, judge_precision)
         self.f_fallback = md.add_field(teacher, fallback_precision)
+        self.f_student.label = f"student-{student_precision}"
+        self.f_judge.label = f"teacher-{judge_precision}"
+        self.f_fallback.label = f"teacher-{fallback_precision}"
         # the list may have grown: forces of the driver at the current positions
         md.compute(md.force, "full")
         self.dt = float(time_step)

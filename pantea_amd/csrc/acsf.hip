@@ -46,8 +46,12 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // ---- precision-specific math -------------------------------------------------
+// ex2k(x): exponential of x scaled by kexp (FP32: v_exp_f32 = 2^x with the
+// log2(e) factor folded into the caller's constants; FP64: libm exp).
 template <typename R> struct M;
 template <> struct M<double> {
+    static constexpr double kexp = 1.0;
+    static __device__ __forceinline__ double ex2k(double x) { return exp(x); }
     static __device__ __forceinline__ double ex(double x) { return exp(x); }
     static __device__ __forceinline__ double th(double x) { return tanh(x); }
     static __device__ __forceinline__ double co(double x) { return cos(x); }
@@ -55,19 +59,30 @@ template <> struct M<double> {
     static __device__ __forceinline__ double sq(double x) { return sqrt(x); }
     static __device__ __forceinline__ double pw(double b, double z) { return pow(b, z); }
     static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+    // r and 1/r from r^2 > 0
+    static __device__ __forceinline__ void root(double r2, double& r, double& ir) {
+        r = sqrt(r2);
+        ir = 1.0 / r;
+    }
 };
 template <> struct M<float> {
-    static __device__ __forceinline__ float ex(float x) { return __expf(x); }
+    static constexpr float kexp = 1.4426950408889634f;  // log2(e)
+    static __device__ __forceinline__ float ex2k(float x) { return __builtin_amdgcn_exp2f(x); }
+    static __device__ __forceinline__ float ex(float x) { return __builtin_amdgcn_exp2f(x * kexp); }
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
     static __device__ __forceinline__ float th(float x) {
-        // tanh(u) = (1 - e^{-2u}) / (1 + e^{-2u}); used for u in [0, 1]
-        const float e = __expf(-2.0f * x);
+        // tanh(u) = (1 - e^{-2u}) / (1 + e^{-2u}); used for u in [-1, 1]
+        const float e = __builtin_amdgcn_exp2f(x * (-2.0f * kexp));
         return (1.0f - e) * rcp(1.0f + e);
     }
     static __device__ __forceinline__ float co(float x) { return __cosf(x); }
     static __device__ __forceinline__ float si(float x) { return __sinf(x); }
-    static __device__ __forceinline__ float sq(float x) { return __builtin_sqrtf(x); }
+    static __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
     static __device__ __forceinline__ float pw(float b, float z) { return powf(b, z); }
+    static __device__ __forceinline__ void root(float r2, float& r, float& ir) {
+        ir = __builtin_amdgcn_rsqf(r2);
+        r = r2 * ir;
+    }
 };
 
 // Cutoff value and derivative (descriptors/acsf/cutoff.py:66-110).
@@ -110,6 +125,28 @@ __device__ __forceinline__ void cutoff(int type, R rc, R inv_rc, R tanh_pre, R r
     f = R(0); df = R(0);
 }
 
+// Cutoff specialised per group: CUTK_TANH (tanh / tanhu, cpre = prefactor),
+// CUTK_COS, or the generic switch.  Branch-free for the first two.
+enum { CUTK_TANH = 0, CUTK_COS = 1, CUTK_GEN = 2 };
+
+template <typename R, int CUT>
+__device__ __forceinline__ void cutoff_k(int type, R rc, R irc, R cpre, R tanh_pre, R r, R& f, R& df) {
+    if (CUT == CUTK_TANH) {
+        const R t = M<R>::th(R(1) - r * irc);
+        const R t2 = t * t;
+        const bool in = r < rc;
+        f = in ? cpre * t2 * t : R(0);
+        df = in ? R(-3) * cpre * t2 * (R(1) - t2) * irc : R(0);
+    } else if (CUT == CUTK_COS) {
+        const R x = R(M_PI) * r * irc;
+        const bool in = r < rc;
+        f = in ? R(0.5) * (M<R>::co(x) + R(1)) : R(0);
+        df = in ? R(-0.5 * M_PI) * irc * M<R>::si(x) : R(0);
+    } else {
+        cutoff<R>(type, rc, irc, tanh_pre, r, f, df);
+    }
+}
+
 // pre (1 + lam c)^zeta and its derivative w.r.t. c (angular.py:60-65).
 template <typename R>
 __device__ __forceinline__ void angular_term(R c, R lam, R zeta, int izeta, R pre, R& v, R& dv) {
@@ -133,14 +170,20 @@ __device__ __forceinline__ void angular_term(R c, R lam, R zeta, int izeta, R pr
     }
 }
 
+// Angular-term evaluation specialised per group: ANGK_Z1 (every zeta = 1),
+// ANGK_P4 (every zeta in {1, 2, 4}) or the generic square-and-multiply / pow.
+enum { ANGK_Z1 = 0, ANGK_P4 = 1, ANGK_GEN = 2 };
+
+template <typename R> struct alignas(4 * sizeof(R)) V4 { R x, y, z, w; };
+
 // Live-pair queue of one wavefront: (a | b << 16) indices of pairs that
-// pass the r_jk screen.
-constexpr int kQueue = 256;
+// pass the r_jk screen.  Batches of 64 are taken as soon as they are full.
+constexpr int kQueue = 128;
 
 template <typename R>
 struct Stage {
-    R* rx; R* ry; R* rz; R* r; R* ir; R* r2;
-    R* f; R* df;   // cutoff of the current group and its derivative
+    V4<R>* P;      // (x, y, z, 1/r) of R_ij = r_i - r_j
+    V4<R>* Q;      // (r, r^2, fc, dfc/dr) -- cutoff of the current group
     R* D;          // [3][cap] TAN: delta_i - delta_j
     double* DA;    // [3][cap] ADJ: per-edge accumulators (FP64: fast LDS atomics)
     int* el; int* slot;
@@ -153,30 +196,23 @@ __host__ __device__ constexpr size_t stage_bytes_per_nbr() {
            2 * sizeof(int);
 }
 
+// cap is a multiple of 4; the per-wave area is a multiple of 32 bytes
 template <typename R, int MODE>
 __host__ __device__ constexpr size_t stage_bytes(int cap) {
-    return (size_t)cap * stage_bytes_per_nbr<R, MODE>() + kQueue * sizeof(unsigned);
+    return ((size_t)cap * stage_bytes_per_nbr<R, MODE>() + kQueue * sizeof(unsigned) + 31) / 32 * 32;
 }
 
 template <typename R, int MODE>
 __device__ __forceinline__ Stage<R> carve(char* base, int cap) {
     Stage<R> s;
-    double* dp = reinterpret_cast<double*>(base);
-    if (MODE == MODE_ADJ) { s.DA = dp; dp += 3 * cap; } else { s.DA = nullptr; }
-    R* p = reinterpret_cast<R*>(dp);
-    s.rx = p; p += cap;
-    s.ry = p; p += cap;
-    s.rz = p; p += cap;
-    s.r = p; p += cap;
-    s.ir = p; p += cap;
-    s.r2 = p; p += cap;
-    s.f = p; p += cap;
-    s.df = p; p += cap;
-    if (MODE == MODE_TAN) { s.D = p; p += 3 * cap; } else { s.D = nullptr; }
-    int* q = reinterpret_cast<int*>(p);
-    s.el = q; q += cap;
-    s.slot = q; q += cap;
-    s.queue = reinterpret_cast<unsigned*>(q);
+    char* p = base;
+    if (MODE == MODE_ADJ) { s.DA = reinterpret_cast<double*>(p); p += 3 * cap * sizeof(double); } else { s.DA = nullptr; }
+    s.P = reinterpret_cast<V4<R>*>(p); p += cap * sizeof(V4<R>);
+    s.Q = reinterpret_cast<V4<R>*>(p); p += cap * sizeof(V4<R>);
+    if (MODE == MODE_TAN) { s.D = reinterpret_cast<R*>(p); p += 3 * cap * sizeof(R); } else { s.D = nullptr; }
+    s.el = reinterpret_cast<int*>(p); p += cap * sizeof(int);
+    s.slot = reinterpret_cast<int*>(p); p += cap * sizeof(int);
+    s.queue = reinterpret_cast<unsigned*>(p);
     return s;
 }
 
@@ -189,6 +225,13 @@ __device__ __forceinline__ R mic_r(R d, R L, R half) {
     d = (d > half) ? d - L : d;
     d = (d < -half) ? d + L : d;
     return d;
+}
+
+template <typename R>
+__device__ __forceinline__ void mic3(const BoxArg& b, R& x, R& y, R& z) {
+    x = mic_r(x, (R)b.L[0], (R)b.half[0]);
+    y = mic_r(y, (R)b.L[1], (R)b.half[1]);
+    z = mic_r(z, (R)b.L[2], (R)b.half[2]);
 }
 
 // Number of staged neighbours with type < t (rows are type-sorted).
@@ -221,9 +264,227 @@ struct KArgs {
     int cap;
 };
 
+// Uniform description of one angular group inside the kernel.
+template <typename R>
+struct AngGroup {
+    int bj, nj, bk, nk;
+    bool same, g3, mic;
+    int n_eta, n_ang, cut;
+    R rc, irc, cpre, tanh_pre;
+    const BoxArg* box;
+    R keta[PT_AG_ETA];     // -eta * kexp: exp(-eta S) = ex2k(keta S)
+    R lam[PT_AG_ANG], zeta[PT_AG_ANG], pre[PT_AG_ANG], dpre[PT_AG_ANG];
+    int izeta[PT_AG_ANG];
+};
+
+template <typename R, int ANG, bool DER>
+__device__ __forceinline__ void ang_eval(const AngGroup<R>& X, R cs, R (&v)[PT_AG_ANG], R (&dv)[PT_AG_ANG]) {
+    // terms q >= n_ang carry pre = dpre = lam = 0 and izeta = 1, so they are 0
+#pragma unroll
+    for (int q = 0; q < PT_AG_ANG; ++q) {
+        if (ANG == ANGK_Z1) {
+            v[q] = X.pre[q] * (R(1) + X.lam[q] * cs);
+            dv[q] = DER ? X.dpre[q] : R(0);
+        } else if (ANG == ANGK_P4) {
+            const R b = R(1) + X.lam[q] * cs;
+            const R b2 = b * b;
+            const int z = X.izeta[q];
+            v[q] = X.pre[q] * (z == 1 ? b : (z == 2 ? b2 : b2 * b2));
+            dv[q] = DER ? X.dpre[q] * (z == 1 ? R(1) : (z == 2 ? b : b * b2)) : R(0);
+        } else {
+            angular_term<R>(cs, X.lam[q], X.zeta[q], X.izeta[q], X.pre[q], v[q], dv[q]);
+        }
+    }
+}
+
+// All unordered pairs {j, k} of one angular group: screening on r_jk, a
+// per-wave queue of live pairs, and the pair body in full-wave batches.
+template <typename R, int MODE, int CUT, int ANG>
+__device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<R>& X,
+                                              const R (&W)[PT_AG_ETA][PT_AG_ANG],
+                                              R (&acc)[PT_AG_ETA][PT_AG_ANG], R& cgx, R& cgy, R& cgz,
+                                              int cap) {
+    const int lane = wave_lane();
+    // cutoff values of the group's neighbours
+    for (int pass = 0; pass < (X.same ? 1 : 2); ++pass) {
+        const int sb = pass == 0 ? X.bj : X.bk, sn = pass == 0 ? X.nj : X.nk;
+        for (int s = sb + lane; s < sb + sn; s += kWave) {
+            R f, df;
+            cutoff_k<R, CUT>(X.cut, X.rc, X.irc, X.cpre, X.tanh_pre, S.Q[s].x, f, df);
+            S.Q[s].z = f;
+            S.Q[s].w = df;
+        }
+    }
+    wave_sync();
+
+    // one live pair (ia, ib)
+    auto body = [&](const int ia, const int ib) {
+        const V4<R> pa = S.P[ia], pb = S.P[ib];
+        const V4<R> qa = S.Q[ia], qb = S.Q[ib];
+        R jx = pa.x - pb.x, jy = pa.y - pb.y, jz = pa.z - pb.z;
+        if (X.mic) mic3(*X.box, jx, jy, jz);
+        const R rjk2 = jx * jx + jy * jy + jz * jz;   // > 0 (screened)
+        R rjk, irjk;
+        M<R>::root(rjk2, rjk, irjk);
+        R fjk = R(1), dfjk = R(0);
+        if (X.g3) cutoff_k<R, CUT>(X.cut, X.rc, X.irc, X.cpre, X.tanh_pre, rjk, fjk, dfjk);
+        const R Ssum = X.g3 ? qa.y + qb.y + rjk2 : qa.y + qb.y;
+        const R F = qa.z * qb.z * fjk;
+        const R iab = pa.w * pb.w;
+        const R cs = (pa.x * pb.x + pa.y * pb.y + pa.z * pb.z) * iab;
+        R ang[PT_AG_ANG], dang[PT_AG_ANG];
+        ang_eval<R, ANG, MODE != MODE_FWD>(X, cs, ang, dang);
+        if (MODE == MODE_FWD) {
+#pragma unroll
+            for (int e = 0; e < PT_AG_ETA; ++e) {
+                if (e < X.n_eta) {
+                    const R rad = M<R>::ex2k(X.keta[e] * Ssum) * F;
+#pragma unroll
+                    for (int q = 0; q < PT_AG_ANG; ++q) acc[e][q] += rad * ang[q];
+                }
+            }
+            return;
+        }
+        // derivative building blocks
+        const R ra = qa.x, rb = qb.x;
+        const R ira = pa.w, irb = pb.w;
+        const R ira2 = ira * ira, irb2 = irb * irb;
+        const R cax = pb.x * iab - cs * pa.x * ira2, cay = pb.y * iab - cs * pa.y * ira2,
+                caz = pb.z * iab - cs * pa.z * ira2;  // d cos / d R_ij
+        const R cbx = pa.x * iab - cs * pb.x * irb2, cby = pa.y * iab - cs * pb.y * irb2,
+                cbz = pa.z * iab - cs * pb.z * irb2;  // d cos / d R_ik
+        if (!X.g3) irjk = R(0);
+        // d F / d r_ij etc. (without the exp factor)
+        const R Fa = qa.w * qb.z * fjk, Fb = qa.z * qb.w * fjk, Fj = qa.z * qb.z * dfjk;
+        if (MODE == MODE_ADJ) {
+            R SP = R(0), SKP = R(0), SV = R(0);
+#pragma unroll
+            for (int e = 0; e < PT_AG_ETA; ++e) {
+                if (e < X.n_eta) {
+                    const R ex = M<R>::ex2k(X.keta[e] * Ssum);
+                    R U = R(0), V = R(0);
+#pragma unroll
+                    for (int q = 0; q < PT_AG_ANG; ++q) { U += W[e][q] * ang[q]; V += W[e][q] * dang[q]; }
+                    const R P = ex * U;
+                    SP += P;
+                    SKP += X.keta[e] * P;
+                    SV += ex * V;
+                }
+            }
+            // dT/dr = d/dr [exp(-eta S) F]: F' sum_e P_e - 2 r F sum_e eta_e P_e,
+            // sum_e eta_e P_e = -SKP / kexp
+            const R F2 = F * R(2.0 / M<R>::kexp);
+            const R d_rij = Fa * SP + ra * F2 * SKP;
+            const R d_rik = Fb * SP + rb * F2 * SKP;
+            const R d_rjk = X.g3 ? (Fj * SP + rjk * F2 * SKP) : R(0);
+            const R d_c = F * SV;
+            const R sa = d_rij * ira, sb = d_rik * irb, sj = d_rjk * irjk;
+            const R gax = sa * pa.x + d_c * cax + sj * jx;
+            const R gay = sa * pa.y + d_c * cay + sj * jy;
+            const R gaz = sa * pa.z + d_c * caz + sj * jz;
+            const R gbx = sb * pb.x + d_c * cbx - sj * jx;
+            const R gby = sb * pb.y + d_c * cby - sj * jy;
+            const R gbz = sb * pb.z + d_c * cbz - sj * jz;
+            cgx += gax + gbx; cgy += gay + gby; cgz += gaz + gbz;
+            lds_add(&S.DA[ia], -(double)gax); lds_add(&S.DA[cap + ia], -(double)gay);
+            lds_add(&S.DA[2 * cap + ia], -(double)gaz);
+            lds_add(&S.DA[ib], -(double)gbx); lds_add(&S.DA[cap + ib], -(double)gby);
+            lds_add(&S.DA[2 * cap + ib], -(double)gbz);
+        } else {  // MODE_TAN
+            const R ax = S.D[ia], ay = S.D[cap + ia], az = S.D[2 * cap + ia];
+            const R bx = S.D[ib], by = S.D[cap + ib], bz = S.D[2 * cap + ib];
+            const R tij = (pa.x * ax + pa.y * ay + pa.z * az) * ira;
+            const R tik = (pb.x * bx + pb.y * by + pb.z * bz) * irb;
+            const R tjk = X.g3 ? (jx * (ax - bx) + jy * (ay - by) + jz * (az - bz)) * irjk : R(0);
+            const R cdot = cax * ax + cay * ay + caz * az + cbx * bx + cby * by + cbz * bz;
+            // d/dt [exp(-eta S) F] = exp(-eta S) (X0 - 2 eta X1)
+            const R X0 = Fa * tij + Fb * tik + Fj * tjk;
+            const R X1 = F * (ra * tij + rb * tik + rjk * tjk);
+#pragma unroll
+            for (int e = 0; e < PT_AG_ETA; ++e) {
+                if (e < X.n_eta) {
+                    const R ex = M<R>::ex2k(X.keta[e] * Ssum);
+                    const R rdot = ex * (X0 + R(2.0 / M<R>::kexp) * X.keta[e] * X1);
+                    const R radc = ex * F * cdot;
+#pragma unroll
+                    for (int q = 0; q < PT_AG_ANG; ++q) acc[e][q] += rdot * ang[q] + radc * dang[q];
+                }
+            }
+        }
+    };
+
+    // Pair enumeration, ordered so that the 64 lanes of one step touch
+    // distinct neighbour slots (few LDS atomic conflicts in the adjoint):
+    //  - cross segments: lanes run over the shorter segment q, the other
+    //    index is the diagonal k = (q + d) mod n_long, d = 0 .. n_long-1;
+    //  - one segment: k = (q + 1 + d) mod n for d < (n-1)/2, plus the
+    //    n/2 pairs (q, q + n/2) when n is even -- each unordered pair once.
+    // Pair p = d * nrow + q; each lane advances p by 64 per step.
+    const bool jrow = X.nj <= X.nk;
+    const int nrow = X.same ? X.nj : (jrow ? X.nj : X.nk);
+    const int ncol = X.same ? X.nj : (jrow ? X.nk : X.nj);
+    const int brow = X.same ? X.bj : (jrow ? X.bj : X.bk);
+    const int bcol = X.same ? X.bj : (jrow ? X.bk : X.bj);
+    const int h = X.same ? (X.nj - 1) / 2 : ncol;
+    const int p1 = nrow * h;
+    const int npairs = X.same ? p1 + ((X.nj % 2 == 0) ? X.nj / 2 : 0) : p1;
+    if (npairs <= 0) return;
+    const int d64 = uniform(kWave / nrow), r64 = uniform(kWave - d64 * nrow);
+    int d = lane / nrow, q = lane - (lane / nrow) * nrow;
+    // screen r_jk^2 < rc^2 (loosened by 1e-4 relative); the exact r_jk < rc
+    // test is the cutoff's own (it is 0 beyond rc)
+    const R rc2 = X.rc * X.rc * R(1.0001);
+    int qn = 0, base = 0;
+    while (true) {
+        if (base < npairs) {
+            const int p = base + lane;
+            bool live = false;
+            int ia = 0, ib = 0;
+            if (p < npairs) {
+                if (p < p1) {
+                    int k = X.same ? q + 1 + d : q + d;
+                    if (k >= ncol) k -= ncol;
+                    ia = brow + q;
+                    ib = bcol + k;
+                } else {
+                    const int t = p - p1;
+                    ia = X.bj + t;
+                    ib = X.bj + t + X.nj / 2;
+                }
+                const V4<R> pa = S.P[ia], pb = S.P[ib];
+                R jx = pa.x - pb.x, jy = pa.y - pb.y, jz = pa.z - pb.z;
+                if (X.mic) mic3(*X.box, jx, jy, jz);
+                const R r2 = jx * jx + jy * jy + jz * jz;
+                live = (r2 > R(0)) && (!X.g3 || r2 < rc2);
+            }
+            const unsigned long long bal = __ballot(live);
+            if (live) S.queue[qn + lanes_below(bal)] = (unsigned)ia | ((unsigned)ib << 16);
+            qn += __popcll(bal);
+            base += kWave;
+            q += r64;
+            d += d64;
+            if (q >= nrow) { q -= nrow; ++d; }
+        }
+        const bool last = base >= npairs;
+        if (qn >= kWave || (last && qn > 0)) {
+            wave_sync();
+            const int take = qn < kWave ? qn : kWave;
+            const unsigned e = lane < take ? S.queue[lane] : 0u;
+            const int rem = qn - take;
+            const unsigned mv = lane < rem ? S.queue[kWave + lane] : 0u;
+            wave_sync();
+            if (lane < rem) S.queue[lane] = mv;
+            qn = rem;
+            if (lane < take) body((int)(e & 0xffffu), (int)(e >> 16));
+        }
+        if (last && qn == 0) break;
+    }
+    wave_sync();
+}
+
 template <typename R, int MODE>
 __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
-    extern __shared__ __align__(16) char smem[];
+    extern __shared__ __align__(32) char smem[];
     const int lane = wave_lane();
     const int wv = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
@@ -238,8 +499,6 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
     const int len = (int)(a.row_off[i + 1] - off);
     const double rcmax = tab->rc_max;
     const R tanh_pre = (R)tab->tanh_pre;
-    const R Lx = (R)a.box.L[0], Ly = (R)a.box.L[1], Lz = (R)a.box.L[2];
-    const R Hx = (R)a.box.half[0], Hy = (R)a.box.half[1], Hz = (R)a.box.half[2];
     // |R_ij - R_ik| <= 2 rc_max: the image shift can only act when L/2 < 2 rc_max
     const double lim = 4.0 * rcmax * (1.0 + 1e-9);
     const bool mic_jk = a.box.periodic && (!(a.box.L[0] > lim) || !(a.box.L[1] > lim) || !(a.box.L[2] > lim));
@@ -267,9 +526,11 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
         const unsigned long long bal = __ballot(keep);
         if (keep) {
             const int s = m + lanes_below(bal);
-            S.rx[s] = (R)dx; S.ry[s] = (R)dy; S.rz[s] = (R)dz; S.r[s] = (R)r;
-            S.ir[s] = (R)(1.0 / r);
-            S.r2[s] = (R)r * (R)r;
+            V4<R> pv, qv;
+            pv.x = (R)dx; pv.y = (R)dy; pv.z = (R)dz; pv.w = (R)(1.0 / r);
+            qv.x = (R)r; qv.y = (R)r * (R)r; qv.z = R(0); qv.w = R(0);
+            S.P[s] = pv;
+            S.Q[s] = qv;
             S.el[s] = (int)tj;
             S.slot[s] = t;
             if (MODE == MODE_ADJ) { S.DA[s] = 0.0; S.DA[cap + s] = 0.0; S.DA[2 * cap + s] = 0.0; }
@@ -302,35 +563,39 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
         const int nfun = uniform(G->n);
         const int cut = uniform(G->cut);
         const R rc = (R)G->rc, irc = (R)(1.0 / G->rc);
-        R wgt[PT_RG_MAX], acc[PT_RG_MAX], eta[PT_RG_MAX], rs[PT_RG_MAX];
+        R wgt[PT_RG_MAX], acc[PT_RG_MAX], keta[PT_RG_MAX], eta[PT_RG_MAX], rs[PT_RG_MAX];
 #pragma unroll
         for (int q = 0; q < PT_RG_MAX; ++q) {
             acc[q] = R(0);
             eta[q] = (q < nfun) ? (R)G->eta[q] : R(0);
+            keta[q] = -eta[q] * M<R>::kexp;
             rs[q] = (q < nfun) ? (R)G->rs[q] : R(0);
             wgt[q] = (MODE == MODE_ADJ && q < nfun) ? w[(size_t)c * a.ldw + G->col[q]] : R(0);
         }
         for (int s = b + lane; s < e; s += kWave) {
-            const R r = S.r[s];
+            const V4<R> qv = S.Q[s];
+            const R r = qv.x;
             R f, df;
             cutoff<R>(cut, rc, irc, tanh_pre, r, f, df);
             R rdot = R(0);
+            V4<R> pv;
+            if (MODE != MODE_FWD) pv = S.P[s];
             if (MODE == MODE_TAN)
-                rdot = (S.rx[s] * S.D[s] + S.ry[s] * S.D[cap + s] + S.rz[s] * S.D[2 * cap + s]) * S.ir[s];
+                rdot = (pv.x * S.D[s] + pv.y * S.D[cap + s] + pv.z * S.D[2 * cap + s]) * pv.w;
             R dsum = R(0);
 #pragma unroll
             for (int q = 0; q < PT_RG_MAX; ++q) {
                 if (q < nfun) {
                     const R d = r - rs[q];
-                    const R ex = M<R>::ex(-eta[q] * d * d);
+                    const R ex = M<R>::ex2k(keta[q] * d * d);
                     if (MODE == MODE_FWD) acc[q] += ex * f;
                     if (MODE == MODE_ADJ) dsum += wgt[q] * ex * (df - R(2) * eta[q] * d * f);
                     if (MODE == MODE_TAN) acc[q] += ex * (df - R(2) * eta[q] * d * f) * rdot;
                 }
             }
             if (MODE == MODE_ADJ) {
-                const R sc = dsum * S.ir[s];
-                const R gx = sc * S.rx[s], gy = sc * S.ry[s], gz = sc * S.rz[s];
+                const R sc = dsum * pv.w;
+                const R gx = sc * pv.x, gy = sc * pv.y, gz = sc * pv.z;
                 cgx += gx; cgy += gy; cgz += gz;
                 S.DA[s] -= (double)gx; S.DA[cap + s] -= (double)gy; S.DA[2 * cap + s] -= (double)gz;
             }
@@ -351,201 +616,64 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
     const int n_ag = uniform(tab->n_ag);
     for (int g = 0; g < n_ag; ++g) {
         const pt_angular_group* G = &tab->ag[g];
+        AngGroup<R> X;
         const int ej = uniform(G->ej), ek = uniform(G->ek);
-        const int bj = count_below(S, m, ej), nj = count_below(S, m, ej + 1) - bj;
-        const int bk = count_below(S, m, ek), nk = count_below(S, m, ek + 1) - bk;
-        const int cut = uniform(G->cut);
-        const bool g3 = uniform(G->kind) == 3;
-        const int n_eta = uniform(G->n_eta), n_ang = uniform(G->n_ang);
-        const R rc = (R)G->rc, irc = (R)(1.0 / G->rc);
-
-        // cutoff values of the group's neighbours
-        for (int pass = 0; pass < (ej == ek ? 1 : 2); ++pass) {
-            const int sb = pass == 0 ? bj : bk, sn = pass == 0 ? nj : nk;
-            for (int s = sb + lane; s < sb + sn; s += kWave) {
-                R f, df;
-                cutoff<R>(cut, rc, irc, tanh_pre, S.r[s], f, df);
-                S.f[s] = f;
-                S.df[s] = df;
-            }
-        }
-
-        R eta[PT_AG_ETA], W[PT_AG_ETA][PT_AG_ANG], acc[PT_AG_ETA][PT_AG_ANG];
+        X.bj = count_below(S, m, ej);
+        X.nj = count_below(S, m, ej + 1) - X.bj;
+        X.bk = count_below(S, m, ek);
+        X.nk = count_below(S, m, ek + 1) - X.bk;
+        X.same = ej == ek;
+        X.g3 = uniform(G->kind) == 3;
+        X.mic = mic_jk;
+        X.n_eta = uniform(G->n_eta);
+        X.n_ang = uniform(G->n_ang);
+        X.cut = uniform(G->cut);
+        X.rc = (R)G->rc;
+        X.irc = (R)(1.0 / G->rc);
+        X.cpre = X.cut == PT_CUT_TANH ? tanh_pre : R(1);
+        X.tanh_pre = tanh_pre;
+        X.box = &a.box;
+        R W[PT_AG_ETA][PT_AG_ANG], acc[PT_AG_ETA][PT_AG_ANG];
 #pragma unroll
         for (int e = 0; e < PT_AG_ETA; ++e) {
-            eta[e] = (e < n_eta) ? (R)G->eta[e] : R(0);
+            X.keta[e] = (e < X.n_eta) ? (R)(-G->eta[e] * M<R>::kexp) : R(0);
 #pragma unroll
             for (int q = 0; q < PT_AG_ANG; ++q) {
                 acc[e][q] = R(0);
                 const int col = G->col[e * PT_AG_ANG + q];
-                W[e][q] = (MODE == MODE_ADJ && e < n_eta && q < n_ang && col >= 0)
+                W[e][q] = (MODE == MODE_ADJ && e < X.n_eta && q < X.n_ang && col >= 0)
                               ? w[(size_t)c * a.ldw + col]
                               : R(0);
             }
         }
-        R lam[PT_AG_ANG], zeta[PT_AG_ANG], pre[PT_AG_ANG];
-        int izeta[PT_AG_ANG];
+        bool z1 = true, p4 = true;
 #pragma unroll
         for (int q = 0; q < PT_AG_ANG; ++q) {
-            lam[q] = (R)G->lam[q]; zeta[q] = (R)G->zeta[q]; pre[q] = (R)G->pre[q];
-            izeta[q] = uniform(G->izeta[q]);
+            const bool on = q < X.n_ang;
+            X.lam[q] = on ? (R)G->lam[q] : R(0);
+            X.zeta[q] = on ? (R)G->zeta[q] : R(1);
+            X.pre[q] = on ? (R)G->pre[q] : R(0);
+            X.izeta[q] = on ? uniform(G->izeta[q]) : 1;
+            X.dpre[q] = on ? (R)(G->pre[q] * G->zeta[q] * G->lam[q]) : R(0);
+            z1 = z1 && X.izeta[q] == 1;
+            p4 = p4 && (X.izeta[q] == 1 || X.izeta[q] == 2 || X.izeta[q] == 4);
         }
+        const int ck = (X.cut == PT_CUT_TANH || X.cut == PT_CUT_TANHU) ? CUTK_TANH
+                       : (X.cut == PT_CUT_COS ? CUTK_COS : CUTK_GEN);
+        const int ak = z1 ? ANGK_Z1 : (p4 ? ANGK_P4 : ANGK_GEN);
         wave_sync();
-
-        // one live pair (ia, ib)
-        auto pair_body = [&](const int ia, const int ib) {
-            const R xa = S.rx[ia], ya = S.ry[ia], za = S.rz[ia], ira = S.ir[ia];
-            const R xb = S.rx[ib], yb = S.ry[ib], zb = S.rz[ib], irb = S.ir[ib];
-            R jx = xa - xb, jy = ya - yb, jz = za - zb;
-            if (mic_jk) { jx = mic_r(jx, Lx, Hx); jy = mic_r(jy, Ly, Hy); jz = mic_r(jz, Lz, Hz); }
-            const R rjk2 = jx * jx + jy * jy + jz * jz;
-            if (!(rjk2 > R(0))) return;
-            const R rjk = M<R>::sq(rjk2);
-            R fjk = R(1), dfjk = R(0);
-            if (g3) {
-                if (!(rjk < rc)) return;
-                cutoff<R>(cut, rc, irc, tanh_pre, rjk, fjk, dfjk);
-            }
-            const R fa = S.f[ia], fb = S.f[ib];
-            const R Ssum = g3 ? S.r2[ia] + S.r2[ib] + rjk2 : S.r2[ia] + S.r2[ib];
-            const R F = fa * fb * fjk;
-            const R iab = ira * irb;
-            const R cs = (xa * xb + ya * yb + za * zb) * iab;
-            R ang[PT_AG_ANG], dang[PT_AG_ANG];
-#pragma unroll
-            for (int q = 0; q < PT_AG_ANG; ++q) {
-                ang[q] = R(0); dang[q] = R(0);
-                if (q < n_ang) angular_term<R>(cs, lam[q], zeta[q], izeta[q], pre[q], ang[q], dang[q]);
-            }
-            if (MODE == MODE_FWD) {
-#pragma unroll
-                for (int e = 0; e < PT_AG_ETA; ++e) {
-                    if (e < n_eta) {
-                        const R rad = M<R>::ex(-eta[e] * Ssum) * F;
-#pragma unroll
-                        for (int q = 0; q < PT_AG_ANG; ++q) acc[e][q] += rad * ang[q];
-                    }
-                }
-                return;
-            }
-            // derivative building blocks
-            const R dfa = S.df[ia], dfb = S.df[ib];
-            const R ra = S.r[ia], rb = S.r[ib];
-            const R ira2 = ira * ira, irb2 = irb * irb;
-            const R cax = xb * iab - cs * xa * ira2, cay = yb * iab - cs * ya * ira2,
-                    caz = zb * iab - cs * za * ira2;  // d cos / d R_ij
-            const R cbx = xa * iab - cs * xb * irb2, cby = ya * iab - cs * yb * irb2,
-                    cbz = za * iab - cs * zb * irb2;  // d cos / d R_ik
-            const R irjk = g3 ? M<R>::rcp(rjk) : R(0);
-            // d F / d r_ij etc. (without the exp factor)
-            const R Fa = dfa * fb * fjk, Fb = fa * dfb * fjk, Fj = fa * fb * dfjk;
-            if (MODE == MODE_ADJ) {
-                R SP = R(0), SEP = R(0), SV = R(0);
-#pragma unroll
-                for (int e = 0; e < PT_AG_ETA; ++e) {
-                    if (e < n_eta) {
-                        const R ex = M<R>::ex(-eta[e] * Ssum);
-                        R U = R(0), V = R(0);
-#pragma unroll
-                        for (int q = 0; q < PT_AG_ANG; ++q) { U += W[e][q] * ang[q]; V += W[e][q] * dang[q]; }
-                        const R P = ex * U;
-                        SP += P;
-                        SEP += eta[e] * P;
-                        SV += ex * V;
-                    }
-                }
-                // dT/dr = d/dr [exp(-eta S) F]: F' sum_e P_e - 2 r F sum_e eta_e P_e
-                const R d_rij = Fa * SP - R(2) * ra * F * SEP;
-                const R d_rik = Fb * SP - R(2) * rb * F * SEP;
-                const R d_rjk = g3 ? (Fj * SP - R(2) * rjk * F * SEP) : R(0);
-                const R d_c = F * SV;
-                const R sa = d_rij * ira, sb = d_rik * irb, sj = d_rjk * irjk;
-                const R gax = sa * xa + d_c * cax + sj * jx;
-                const R gay = sa * ya + d_c * cay + sj * jy;
-                const R gaz = sa * za + d_c * caz + sj * jz;
-                const R gbx = sb * xb + d_c * cbx - sj * jx;
-                const R gby = sb * yb + d_c * cby - sj * jy;
-                const R gbz = sb * zb + d_c * cbz - sj * jz;
-                cgx += gax + gbx; cgy += gay + gby; cgz += gaz + gbz;
-                lds_add(&S.DA[ia], -(double)gax); lds_add(&S.DA[cap + ia], -(double)gay);
-                lds_add(&S.DA[2 * cap + ia], -(double)gaz);
-                lds_add(&S.DA[ib], -(double)gbx); lds_add(&S.DA[cap + ib], -(double)gby);
-                lds_add(&S.DA[2 * cap + ib], -(double)gbz);
-            } else {  // MODE_TAN
-                const R ax = S.D[ia], ay = S.D[cap + ia], az = S.D[2 * cap + ia];
-                const R bx = S.D[ib], by = S.D[cap + ib], bz = S.D[2 * cap + ib];
-                const R tij = (xa * ax + ya * ay + za * az) * ira;
-                const R tik = (xb * bx + yb * by + zb * bz) * irb;
-                const R tjk = g3 ? (jx * (ax - bx) + jy * (ay - by) + jz * (az - bz)) * irjk : R(0);
-                const R cdot = cax * ax + cay * ay + caz * az + cbx * bx + cby * by + cbz * bz;
-                // d/dt [exp(-eta S) F] = exp(-eta S) (X0 - 2 eta X1)
-                const R X0 = Fa * tij + Fb * tik + Fj * tjk;
-                const R X1 = F * (ra * tij + rb * tik + rjk * tjk);
-#pragma unroll
-                for (int e = 0; e < PT_AG_ETA; ++e) {
-                    if (e < n_eta) {
-                        const R ex = M<R>::ex(-eta[e] * Ssum);
-                        const R rdot = ex * (X0 - R(2) * eta[e] * X1);
-                        const R radc = ex * F * cdot;
-#pragma unroll
-                        for (int q = 0; q < PT_AG_ANG; ++q) acc[e][q] += rdot * ang[q] + radc * dang[q];
-                    }
-                }
-            }
-        };
-
-        // Pair enumeration, ordered so that the 64 lanes of one step touch
-        // distinct neighbour slots (no LDS atomic conflicts in the adjoint):
-        //  - cross segments: lanes run over the shorter segment q, the other
-        //    index is the diagonal k = (q + d) mod n_long, d = 0 .. n_long-1;
-        //  - one segment: k = (q + 1 + t) mod n for t < (n-1)/2, plus the
-        //    n/2 pairs (q, q + n/2) when n is even -- each unordered pair once.
-        const bool same = ej == ek;
-        const bool jrow = nj <= nk;
-        const int nrow = same ? nj : (jrow ? nj : nk);
-        const int ncol = same ? nj : (jrow ? nk : nj);
-        const int brow = same ? bj : (jrow ? bj : bk);
-        const int bcol = same ? bj : (jrow ? bk : bj);
-        const int h = same ? (nj - 1) / 2 : ncol;
-        const int p1 = nrow * h;
-        const int npairs = same ? p1 + ((nj % 2 == 0) ? nj / 2 : 0) : p1;
-        const float inv_n = nrow > 0 ? 1.0f / (float)nrow : 0.0f;
-        // screen r_jk^2 < rc^2 (loosened by 1e-4 relative); the exact rjk < rc test is in pair_body
-        const R rc2 = rc * rc * R(1.0001);
-        int qn = 0;
-        for (int base = 0; base < npairs; base += kWave) {
-            const int p = base + lane;
-            bool live = false;
-            int ia = 0, ib = 0;
-            if (p < npairs) {
-                if (p < p1) {
-                    int d = (int)(((float)p + 0.5f) * inv_n);
-                    int q = p - d * nrow;
-                    if (q < 0) { --d; q += nrow; } else if (q >= nrow) { ++d; q -= nrow; }
-                    int k = same ? q + 1 + d : q + d;
-                    if (k >= ncol) k -= ncol;
-                    ia = brow + q; ib = bcol + k;
-                } else {
-                    const int q = p - p1;
-                    ia = bj + q; ib = bj + q + nj / 2;
-                }
-                R jx = S.rx[ia] - S.rx[ib], jy = S.ry[ia] - S.ry[ib], jz = S.rz[ia] - S.rz[ib];
-                if (mic_jk) { jx = mic_r(jx, Lx, Hx); jy = mic_r(jy, Ly, Hy); jz = mic_r(jz, Lz, Hz); }
-                const R r2 = jx * jx + jy * jy + jz * jz;
-                live = (r2 > R(0)) && (!g3 || r2 < rc2);
-            }
-            const unsigned long long bal = __ballot(live);
-            if (live) S.queue[qn + lanes_below(bal)] = (unsigned)ia | ((unsigned)ib << 16);
-            qn += __popcll(bal);
-            if (qn > kQueue - kWave || base + kWave >= npairs) {
-                wave_sync();
-                for (int q = lane; q < qn; q += kWave) {
-                    const unsigned e = S.queue[q];
-                    pair_body((int)(e & 0xffffu), (int)(e >> 16));
-                }
-                qn = 0;
-                wave_sync();
-            }
+        if (MODE == MODE_TAN || sizeof(R) == 8) {
+            angular_group<R, MODE, CUTK_GEN, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
+        } else if (ck == CUTK_TANH) {
+            if (ak == ANGK_Z1) angular_group<R, MODE, CUTK_TANH, ANGK_Z1>(S, X, W, acc, cgx, cgy, cgz, cap);
+            else if (ak == ANGK_P4) angular_group<R, MODE, CUTK_TANH, ANGK_P4>(S, X, W, acc, cgx, cgy, cgz, cap);
+            else angular_group<R, MODE, CUTK_TANH, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
+        } else if (ck == CUTK_COS) {
+            if (ak == ANGK_Z1) angular_group<R, MODE, CUTK_COS, ANGK_Z1>(S, X, W, acc, cgx, cgy, cgz, cap);
+            else if (ak == ANGK_P4) angular_group<R, MODE, CUTK_COS, ANGK_P4>(S, X, W, acc, cgx, cgy, cgz, cap);
+            else angular_group<R, MODE, CUTK_COS, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
+        } else {
+            angular_group<R, MODE, CUTK_GEN, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
         }
 
         if (MODE != MODE_ADJ) {
@@ -559,7 +687,7 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
 #pragma unroll
                     for (int q = 0; q < PT_AG_ANG; ++q) {
                         const int col = G->col[e * PT_AG_ANG + q];
-                        if (e < n_eta && q < n_ang && col >= 0) out[(size_t)c * a.ld + col] = acc[e][q];
+                        if (e < X.n_eta && q < X.n_ang && col >= 0) out[(size_t)c * a.ld + col] = acc[e][q];
                     }
             }
         }
@@ -590,7 +718,7 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
 template <typename R, int MODE>
 int launch(KArgs a, hipStream_t stream) {
     if (a.n_centers <= 0) return PT_OK;
-    a.cap = std::max(a.cap, 1);
+    a.cap = (std::max(a.cap, 1) + 3) / 4 * 4;
     if (a.cap > 65535) {
         set_error("acsf: neighbour rows longer than 65535 entries are not supported");
         return PT_ERR_UNSUPPORTED;

@@ -85,11 +85,55 @@ __device__ __forceinline__ double exact_dist(double& dx, double& dy, double& dz,
 // ---------------------------------------------------------------------------
 // Wave-level reductions.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    return v;
+// Sum over the 64 lanes with DPP moves (no LDS traffic): pairs (xor 1),
+// quads (xor 2), half rows and rows (mirrors), then rows 0+1 into row 1 and
+// 2+3 into row 3 (row_bcast:15), then all into row 3 (row_bcast:31).  The
+// total ends in lane 63 and is returned to every lane.  All 64 lanes must be
+// active.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_f32(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROW_MASK,
+                                                                 0xF, false));
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u & 0xffffffffu), CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, ROW_MASK, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+enum : int {
+    kDppXor1 = 0xB1,      // quad_perm [1, 0, 3, 2]
+    kDppXor2 = 0x4E,      // quad_perm [2, 3, 0, 1]
+    kDppHalfMirror = 0x141,
+    kDppMirror = 0x140,
+    kDppBcast15 = 0x142,
+    kDppBcast31 = 0x143,
+};
+
+__device__ __forceinline__ float wave_sum(float v) {
+    v += dpp_f32<kDppXor1, 0xF>(v);
+    v += dpp_f32<kDppXor2, 0xF>(v);
+    v += dpp_f32<kDppHalfMirror, 0xF>(v);
+    v += dpp_f32<kDppMirror, 0xF>(v);
+    v += dpp_f32<kDppBcast15, 0xA>(v);
+    v += dpp_f32<kDppBcast31, 0xC>(v);
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    v += dpp_f64<kDppXor1, 0xF>(v);
+    v += dpp_f64<kDppXor2, 0xF>(v);
+    v += dpp_f64<kDppHalfMirror, 0xF>(v);
+    v += dpp_f64<kDppMirror, 0xF>(v);
+    v += dpp_f64<kDppBcast15, 0xA>(v);
+    v += dpp_f64<kDppBcast31, 0xC>(v);
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u & 0xffffffffu), 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), 63);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
 __device__ __forceinline__ int wave_lane() { return threadIdx.x & (kWave - 1); }

@@ -161,3 +161,59 @@ def test_training_reduces_loss_and_force_error():
     rmse = math.sqrt(float(((f - F_ref) ** 2).mean()))
     from pantea_amd.units import units
     assert rmse * units.TO_MILLI_ELECTRON_VOLT == pytest.approx(after["force_rmse_mev_per_bohr"], rel=1e-6)
+
+
+def test_trained_student_gains_over_untrained_network():
+    """What the learned part buys: with a teacher the student can represent (the
+    student's descriptors and layer widths, other weights), GPU training on the
+    teacher's own MD data cuts the student's force error several-fold, and in
+    surrogate MD the trained student passes the judge and drives the run, while
+    the same network untrained is rejected at every judge and the teacher drives
+    everything."""
+    import torch
+    from pantea_amd import water
+    from pantea_amd.atoms import Structure
+    from pantea_amd.bench_models import make_nnp, student_acsf
+    from pantea_amd.simulation import BrendsenThermostat
+    from pantea_amd.student import StudentTrainer, teacher_snapshots
+    from pantea_amd.surrogate import SurrogateMD
+    from pantea_amd.units import units
+
+    mev = units.TO_MILLI_ELECTRON_VOLT
+    s0 = Structure.from_dict(water.water_box(64, water.box_for_density(64), seed=21))
+    teacher = make_nnp(student_acsf, (16, 16), seed=11, precision="float64")
+    teacher.fit_scaler([s0])
+    # a smoother teacher (kernels x 0.5) whose scaler the student shares
+    teacher.set_model_params({e: {k: {"kernel": 0.5 * d["kernel"], "bias": d["bias"]} for k, d in prm.items()}
+                              for e, prm in teacher.models_params.items()})
+    student = make_nnp(student_acsf, (16, 16), seed=12, precision="mixed")
+    student.scalers_params = dict(teacher.scalers_params)
+    untrained_params = {e: {k: {p: v.copy() for p, v in d.items()} for k, d in prm.items()}
+                        for e, prm in student.models_params.items()}
+    tr = StudentTrainer(student, precision="mixed", w_energy=1.0, w_force=10.0, lr=5e-3, seed=1)
+    # 4 teacher trajectories of 120 steps (300 K), a snapshot every 5 steps
+    for snap, e, f in teacher_snapshots(teacher, n_structures=4, n_snapshots=96, every=5, min_molecules=48,
+                                        max_molecules=72, seed=3):
+        tr.add(snap, e, f)
+    tr.prepare(fit_scaler=False)
+    idx = np.arange(len(tr.snapshots))
+    val, train = idx[::6], np.setdiff1d(idx, idx[::6])
+    before = tr.evaluate(val)["force_rmse_mev_per_bohr"]
+    tr.train(600, batch=8, train_idx=train)
+    after = tr.evaluate(val)["force_rmse_mev_per_bohr"]
+    print(f"validation force RMSE: untrained {before:.1f}, trained {after:.1f} meV/Bohr")
+    assert after < 0.4 * before
+
+    untrained = make_nnp(student_acsf, (16, 16), seed=12, precision="mixed")
+    untrained.set_model_params(untrained_params)
+    untrained.scalers_params = dict(student.scalers_params)
+    thr = math.sqrt(before * after) / mev
+    runs = {}
+    for name, stu in (("trained", student), ("untrained", untrained)):
+        sur = SurrogateMD(teacher, stu, s0, seed=5, judge_every=10, threshold=thr, fallback_steps=20, skin=1.0,
+                          thermostat=BrendsenThermostat(300.0, 0.5 * units.FROM_FEMTO_SECOND))
+        runs[name] = sur.run(120).as_dict()
+        print(name, runs[name])
+    assert runs["untrained"]["failures"] == runs["untrained"]["judges"] > 0
+    assert runs["trained"]["teacher_share"] < 0.5 * runs["untrained"]["teacher_share"]
+    assert runs["trained"]["student_steps"] - runs["trained"]["discarded_steps"] > 60

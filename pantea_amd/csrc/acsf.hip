@@ -49,20 +49,79 @@ __device__ __forceinline__ void wave_sync() {
 // ex2k(x): exponential of x scaled by kexp (FP32: v_exp_f32 = 2^x with the
 // log2(e) factor folded into the caller's constants; FP64: libm exp).
 template <typename R> struct M;
+// FP64 elementary functions for the pair loops (the kernels' accuracy goal is
+// ~1e-15 relative, far inside the 1e-10 parity bound):
+//  * exp: Tang's table-driven method (ACM TOMS 15(2), 1989): x = (32 m + j)
+//    ln2/32 + r with |r| <= ln2/64 (two-constant Cody-Waite reduction),
+//    e^x = 2^m * 2^(j/32) * e^r, e^r by its degree-6 Taylor polynomial
+//    (remainder < 4e-18); the 32 values 2^(j/32) live in LDS;
+//  * 1/x and 1/sqrt(x): the hardware estimates refined by two Newton steps.
+__constant__ double kExp2Tab[32] = {
+    0x1.0000000000000p+0, 0x1.059b0d3158574p+0, 0x1.0b5586cf9890fp+0, 0x1.11301d0125b51p+0,
+    0x1.172b83c7d517bp+0, 0x1.1d4873168b9aap+0, 0x1.2387a6e756238p+0, 0x1.29e9df51fdee1p+0,
+    0x1.306fe0a31b715p+0, 0x1.371a7373aa9cbp+0, 0x1.3dea64c123422p+0, 0x1.44e086061892dp+0,
+    0x1.4bfdad5362a27p+0, 0x1.5342b569d4f82p+0, 0x1.5ab07dd485429p+0, 0x1.6247eb03a5585p+0,
+    0x1.6a09e667f3bcdp+0, 0x1.71f75e8ec5f74p+0, 0x1.7a11473eb0187p+0, 0x1.82589994cce13p+0,
+    0x1.8ace5422aa0dbp+0, 0x1.93737b0cdc5e5p+0, 0x1.9c49182a3f090p+0, 0x1.a5503b23e255dp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b7f76f2fb5e47p+0, 0x1.c199bdd85529cp+0, 0x1.cb720dcef9069p+0,
+    0x1.d5818dcfba487p+0, 0x1.dfc97337b9b5fp+0, 0x1.ea4afa2a490dap+0, 0x1.f50765b6e4540p+0,
+};
+__shared__ double g_exp2_tab[32];
+
+// every thread of the block calls this before any early exit
+__device__ __forceinline__ void exp2_tab_load() {
+    if (threadIdx.x < 32) g_exp2_tab[threadIdx.x] = kExp2Tab[threadIdx.x];
+    __syncthreads();
+}
+
+__device__ __forceinline__ double exp_tab(double x) {
+    x = fmax(x, -708.0);
+    const double nf = __builtin_rint(x * 0x1.71547652b82fep+5);          // 32 / ln2
+    double r = fma(nf, -0x1.62e42fee00000p-6, x);                       // ln2/32, high part
+    r = fma(nf, -0x1.a39ef35793c76p-38, r);                             // ln2/32, low part
+    const int n = (int)nf;
+    double p = fma(r, 1.0 / 720.0, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return __builtin_amdgcn_ldexp(g_exp2_tab[n & 31] * p, n >> 5);
+}
+
+__device__ __forceinline__ double rcp_nr(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    y = fma(y, fma(-x, y, 1.0), y);
+    y = fma(y, fma(-x, y, 1.0), y);
+    return y;
+}
+
+__device__ __forceinline__ double rsq_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = y * fma(-h, y * y, 1.5);
+    y = y * fma(-h, y * y, 1.5);
+    return y;
+}
+
 template <> struct M<double> {
     static constexpr double kexp = 1.0;
-    static __device__ __forceinline__ double ex2k(double x) { return exp(x); }
+    static __device__ __forceinline__ double ex2k(double x) { return exp_tab(x); }
     static __device__ __forceinline__ double ex(double x) { return exp(x); }
-    static __device__ __forceinline__ double th(double x) { return tanh(x); }
+    static __device__ __forceinline__ double th(double x) {
+        // tanh(u) = (1 - e^{-2u}) / (1 + e^{-2u}); used for u in [-1, 1]
+        const double e = exp_tab(-2.0 * x);
+        return (1.0 - e) * rcp_nr(1.0 + e);
+    }
     static __device__ __forceinline__ double co(double x) { return cos(x); }
     static __device__ __forceinline__ double si(double x) { return sin(x); }
     static __device__ __forceinline__ double sq(double x) { return sqrt(x); }
     static __device__ __forceinline__ double pw(double b, double z) { return pow(b, z); }
-    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+    static __device__ __forceinline__ double rcp(double x) { return rcp_nr(x); }
     // r and 1/r from r^2 > 0
     static __device__ __forceinline__ void root(double r2, double& r, double& ir) {
-        r = sqrt(r2);
-        ir = 1.0 / r;
+        ir = rsq_nr(r2);
+        r = r2 * ir;
     }
 };
 template <> struct M<float> {
@@ -485,6 +544,7 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
 template <typename R, int MODE>
 __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
     extern __shared__ __align__(32) char smem[];
+    if (sizeof(R) == 8) exp2_tab_load();
     const int lane = wave_lane();
     const int wv = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;

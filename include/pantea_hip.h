@@ -142,6 +142,14 @@ int pt_nbr_build(int n, const double *pos, int pos_stride, const int *types, int
                  long long cap, long long *n_pairs, int *max_row,
                  void *ws, size_t ws_bytes, pt_stream_t stream);
 
+/* Mark the closed neighbourhoods of n_sel atoms: flags[s] = 1 and
+ * flags[j] = 1 for every j in the row of s (flags: device, n entries, not
+ * cleared here).  Used to evaluate a potential on the centres whose
+ * descriptors a few atoms' full forces depend on (the sampled judge of the
+ * surrogate MD driver); no counterpart in the reference. */
+int pt_nbr_mark_rows(int n_sel, const int *sel, const long long *row_off, const int *nbr,
+                     unsigned char *flags, pt_stream_t stream);
+
 /* ------------------------------------------------------------------------
  * (b) ACSF forward: descriptor rows of the centres
  *     centers[c] (or center_begin + c when centers == NULL), internal order.

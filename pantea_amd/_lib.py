@@ -95,6 +95,7 @@ _SIGS = {
     "pt_nbr_workspace_bytes": (_c_size, [_c_int, _c_int]),
     "pt_nbr_build": (_c_int, [_c_int, _vp, _c_int, _vp, _c_int, _vp, _c_dbl, _vp, _vp, _vp, _vp, _vp,
                               _c_ll, ctypes.POINTER(_c_ll), ctypes.POINTER(_c_int), _vp, _c_size, _vp]),
+    "pt_nbr_mark_rows": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
     "pt_acsf_forward": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp,
                                  _vp, _c_int, _vp]),
     "pt_acsf_adjoint": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp,

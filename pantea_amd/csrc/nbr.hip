@@ -405,6 +405,17 @@ __global__ __launch_bounds__(256) void k_rev(int n, const long long* __restrict_
     }
 }
 
+__global__ __launch_bounds__(256) void k_mark_rows(int n_sel, const int* __restrict__ sel,
+                                                   const long long* __restrict__ row_off,
+                                                   const int* __restrict__ nbr, unsigned char* __restrict__ flags) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int q = uniform(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (q >= n_sel) return;
+    const int i = sel[q];
+    if (lane == 0) flags[i] = 1;
+    for (long long k = row_off[i] + lane; k < row_off[i + 1]; k += kWave) flags[nbr[k]] = 1;
+}
+
 struct WsLayout {
     size_t keys_a, keys_b, vals_a, vals_b, ghist, cell_start, counts, psum, pmax, misc, total;
 };
@@ -578,4 +589,13 @@ extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int*
         if (int rcode = launch_check("k_rev")) return rcode;
     }
     return PT_OK;
+}
+
+extern "C" int pt_nbr_mark_rows(int n_sel, const int* sel, const long long* row_off, const int* nbr,
+                                unsigned char* flags, pt_stream_t stream) {
+    PT_REQUIRE(n_sel >= 0 && (n_sel == 0 || (sel && row_off && nbr && flags)), "pt_nbr_mark_rows: bad arguments");
+    if (n_sel == 0) return PT_OK;
+    hipLaunchKernelGGL(k_mark_rows, dim3((n_sel + 3) / 4), dim3(256), 0, (hipStream_t)stream, n_sel, sel, row_off,
+                       nbr, flags);
+    return launch_check("k_mark_rows");
 }

@@ -2,10 +2,18 @@
 
 A student NNP drives velocity-Verlet MD.  Every ``judge_every`` steps the
 teacher NNP (FP64 by default) is evaluated on the current positions and its
-forces are compared with the student's (RMSE over all force components).
-When the RMSE exceeds ``threshold`` the run rewinds to the last state that
-passed and the teacher drives the next ``fallback_steps`` steps; then the
-student takes over again from the state the teacher reached.
+forces are compared with the student's (RMSE over all force components, or
+over the atoms of a random sample when ``judge_atoms`` is set).  When the
+RMSE exceeds ``threshold`` the run rewinds to the last state that passed and
+the teacher drives the next ``fallback_steps`` steps; then the student takes
+over again from the state the teacher reached.
+
+A sampled judge evaluates the teacher only where the sampled atoms' full
+forces need it: F_k = -sum_i dE_i/dr_k involves the centres i within the
+cutoff of k, so descriptors, MLP and adjoint run on the closed
+neighbourhoods of the sample (pt_nbr_mark_rows), and the force gather then
+gives exact teacher forces on the sampled atoms.  Its cost is the share of
+atoms in those neighbourhoods instead of the whole system.
 
 Student, judge and fallback potentials share one device state and one
 neighbour list (cutoff = the largest of them + skin), so switching the
@@ -21,6 +29,7 @@ from typing import Optional
 
 import numpy as np
 
+from . import _lib
 from .simulation import BrendsenThermostat, System
 from .units import units
 
@@ -65,7 +74,8 @@ class SurrogateMD:
                  student_precision: str = "mixed", judge_precision: str = "float64",
                  fallback_precision: str = "float64", judge_every: int = 20,
                  threshold: float = 5.0 / units.TO_MILLI_ELECTRON_VOLT, fallback_steps: int = 200,
-                 skin: float = 1.0, thermostat: Optional[BrendsenThermostat] = None):
+                 skin: float = 1.0, thermostat: Optional[BrendsenThermostat] = None,
+                 judge_atoms: Optional[int] = None, judge_seed: int = 0):
         import torch
 
         if integrator not in ("reference", "physical"):
@@ -91,6 +101,9 @@ class SurrogateMD:
         self.threshold = float(threshold)
         self.fallback_len = int(fallback_steps)
         self.f_judge_out = torch.zeros_like(md.force)
+        self.judge_atoms = judge_atoms
+        self.rng = np.random.default_rng(judge_seed)
+        self._flags = None
         self.stats = SurrogateStats()
         self.mode = "student"
         self.fallback_left = 0
@@ -100,12 +113,60 @@ class SurrogateMD:
     # ---- judge -------------------------------------------------------------------
     def judge(self) -> float:
         """Force RMSE (Ha/Bohr) of the driving student against the judge at the current positions."""
+        md = self.md
+        if self.judge_atoms and self.judge_atoms < md.n:
+            sel = np.sort(self.rng.choice(md.n, int(self.judge_atoms), replace=False)).astype(np.int32)
+            f = self.judge_forces(sel)
+            diff = md.force[self.sel_dev] - f
+        else:
+            md.compute(self.f_judge_out, "full", field=self.f_judge)
+            diff = md.force - self.f_judge_out
+        return math.sqrt(float((diff * diff).mean()))
+
+    def judge_forces(self, sel: np.ndarray):
+        """Judge forces -grad E on the atoms ``sel`` (internal order), evaluating only their neighbourhoods."""
         import torch
 
+        from .engine import acsf_adjoint, acsf_forward, force_gather
+
         md = self.md
-        md.compute(self.f_judge_out, "full", field=self.f_judge)
-        diff = md.force - self.f_judge_out
-        return math.sqrt(float((diff * diff).mean()))
+        ff = self.f_judge
+        nl = md.nl
+        dev = md.dev
+        lib = _lib.load()
+        if self._flags is None or self._flags.numel() != md.n:
+            self._flags = torch.zeros(md.n, dtype=torch.uint8, device=dev)
+        self._flags.zero_()
+        self.sel_dev = torch.as_tensor(np.ascontiguousarray(sel, dtype=np.int32), device=dev)
+        _lib.check(lib.pt_nbr_mark_rows(int(sel.size), self.sel_dev.data_ptr(), nl.row_off.data_ptr(),
+                                        nl.nbr.data_ptr(), self._flags.data_ptr(), _lib.stream_ptr()),
+                   "pt_nbr_mark_rows")
+        cent = np.flatnonzero(self._flags.cpu().numpy()).astype(np.int32)
+        edt = torch.float64 if ff.g_prec == _lib.PT_F64 else torch.float32
+        if ff.edge is None or ff.edge.numel() < nl.n_pairs * 3 or ff.edge.dtype != edt:
+            ff.edge = torch.zeros(int(nl.n_pairs * 3 * 1.2) + 3, dtype=edt, device=dev)
+        if not ff.covered:
+            ff.edge.zero_()
+        fdt = ff.G.dtype
+        blocks = []
+        for e, b, cnt in ff._blocks(nl):
+            c = cent[(cent >= b) & (cent < b + cnt)]
+            if c.size:
+                blocks.append((e, torch.as_tensor(c, device=dev)))
+        m = sum(int(c.numel()) for _, c in blocks)
+        G = torch.zeros((m, ff.ld), dtype=fdt, device=dev)
+        dEdG = torch.zeros_like(G)
+        e_atom = torch.zeros(m, dtype=torch.float64, device=dev)
+        o = 0
+        for e, c in blocks:
+            k = int(c.numel())
+            acsf_forward(nl, ff.tables[e], ff.g_prec, k, centers=c, out=G[o:], ld=ff.ld)
+            ff.mlps[e].energy_grad(ff.mlp_mode, k, G[o:], ff.ld, ff.g_prec, e_atom[o:], dEdG[o:])
+            acsf_adjoint(nl, ff.tables[e], ff.g_prec, k, dEdG[o:], ff.ld, centers=c, edge_grad=ff.edge)
+            o += k
+        self.judge_centres = m
+        force_gather(nl, ff.edge, ff.g_prec, out=self.f_judge_out)
+        return self.f_judge_out[self.sel_dev]
 
     # ---- driving -----------------------------------------------------------------
     def _to_student(self) -> None:

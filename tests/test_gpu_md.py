@@ -218,3 +218,32 @@ def test_surrogate_failed_judge_rewinds_to_teacher_trajectory():
                                   skin=1.0)
     MDSimulator(0.5 * units.FROM_FEMTO_SECOND, integrator="physical").run(plain, 25)
     assert float((sur.positions - plain.positions).abs().max()) < 1e-9
+
+
+def test_sampled_judge_forces_equal_full_evaluation():
+    """The sampled judge evaluates the teacher on the sampled atoms' neighbourhoods only; its
+    forces on those atoms equal the full evaluation's (same kernels, same FP64 sums)."""
+    import torch
+    from pantea_amd import water
+    from pantea_amd.atoms import Structure
+    from pantea_amd.bench_models import make_nnp, student_acsf, water_nnp
+    from pantea_amd.surrogate import SurrogateMD
+
+    s = Structure.from_dict(water.water_box(4096, 93.85, seed=7))
+    teacher = water_nnp(s, hidden=(25, 25), seed=7, precision="float64")
+    student = make_nnp(student_acsf, (16, 16), seed=8, precision="mixed")
+    student.fit_scaler([s])
+    sur = SurrogateMD(teacher, student, s, seed=1, judge_every=10, judge_atoms=32, skin=1.0)
+    md = sur.md
+    sel = np.sort(np.random.default_rng(4).choice(md.n, 32, replace=False)).astype(np.int32)
+    f_sub = sur.judge_forces(sel).cpu().numpy()
+    centres = sur.judge_centres
+    full = torch.zeros_like(md.force)
+    md.compute(full, "full", field=sur.f_judge)
+    ref = full.cpu().numpy()[sel]
+    print(f"sampled judge: {centres} of {md.n} centres evaluated")
+    assert centres < 0.6 * md.n
+    np.testing.assert_allclose(f_sub, ref, rtol=0, atol=1e-12 * np.max(np.abs(ref)))
+    st = sur.run(30)   # an untrained student: judged once at step 10, rejected, teacher to the end
+    assert st.judges == st.failures == 1 and st.fallback_steps == 30
+    assert all(math.isfinite(r) for r in st.judge_rmse)

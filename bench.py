@@ -175,7 +175,8 @@ def run_surrogate(teacher, student, s, steps, warmup, args):
 
     sur = SurrogateMD(teacher, student, s, seed=2023, judge_every=args.judge_every,
                       threshold=args.threshold_mev / units.TO_MILLI_ELECTRON_VOLT,
-                      fallback_steps=args.fallback_steps, skin=args.skin, thermostat=_thermostat(args))
+                      fallback_steps=args.fallback_steps, skin=args.skin, thermostat=_thermostat(args),
+                      judge_atoms=args.judge_atoms or None)
     sur.run(warmup)
     _sync()
     before = sur.stats.as_dict()
@@ -211,13 +212,15 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400, help="timed steps of the headline (surrogate) leg")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--teacher-steps", type=int, default=40, help="timed steps of the mixed teacher leg")
-    ap.add_argument("--fp64-steps", type=int, default=10)
+    ap.add_argument("--teacher-steps", type=int, default=100, help="timed steps of the mixed teacher leg")
+    ap.add_argument("--fp64-steps", type=int, default=60)
     ap.add_argument("--skin", type=float, default=1.0)
     ap.add_argument("--tau-fs", type=float, default=0.5, help="Berendsen time constant (0: NVE)")
     ap.add_argument("--judge-every", type=int, default=20)
     ap.add_argument("--fallback-steps", type=int, default=200)
     ap.add_argument("--threshold-mev", type=float, default=5.0)
+    ap.add_argument("--judge-atoms", type=int, default=0,
+                    help="judge on this many sampled atoms (their neighbourhoods only); 0: every atom")
     ap.add_argument("--molecules", type=int, default=32768)
     ap.add_argument("--box", type=float, default=187.7)
     ap.add_argument("--student", default=None, help="student directory (default: the packaged water student)")
@@ -343,9 +346,10 @@ def main() -> int:
         del src, dst
 
     thermo = f"Berendsen 300 K tau {args.tau_fs} fs" if args.tau_fs > 0 else "NVE"
+    judged = f"{args.judge_atoms} sampled atoms" if args.judge_atoms else "all atoms"
     head = ("surrogate: student 34 ACSF 34-16-16-1 mixed drives, FP64 teacher judges every "
-            f"{args.judge_every} steps, > {args.threshold_mev} meV/Bohr -> rewind + {args.fallback_steps} FP64 "
-            "teacher steps") if sur is not None else "mixed-precision teacher (no trained student found)"
+            f"{args.judge_every} steps on {judged}, > {args.threshold_mev} meV/Bohr -> rewind + "
+            f"{args.fallback_steps} FP64 teacher steps") if sur is not None else "mixed-precision teacher (no trained student found)"
     out = {
         "metric": "atom-steps/s, 98k-atom water NNP-MD: mixed+surrogate vs FP64; force RMSE",
         "value": value,

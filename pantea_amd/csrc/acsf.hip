@@ -107,7 +107,7 @@ __device__ __forceinline__ double rsq_nr(double x) {
 template <> struct M<double> {
     static constexpr double kexp = 1.0;
     static __device__ __forceinline__ double ex2k(double x) { return exp_tab(x); }
-    static __device__ __forceinline__ double ex(double x) { return exp(x); }
+    static __device__ __forceinline__ double ex(double x) { return exp_tab(x); }
     static __device__ __forceinline__ double th(double x) {
         // tanh(u) = (1 - e^{-2u}) / (1 + e^{-2u}); used for u in [-1, 1]
         const double e = exp_tab(-2.0 * x);
@@ -722,8 +722,10 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
                        : (X.cut == PT_CUT_COS ? CUTK_COS : CUTK_GEN);
         const int ak = z1 ? ANGK_Z1 : (p4 ? ANGK_P4 : ANGK_GEN);
         wave_sync();
-        if (MODE == MODE_TAN || sizeof(R) == 8) {
+        if (MODE == MODE_TAN || (sizeof(R) == 8 && ck != CUTK_TANH)) {
             angular_group<R, MODE, CUTK_GEN, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
+        } else if (sizeof(R) == 8) {
+            angular_group<R, MODE, CUTK_TANH, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
         } else if (ck == CUTK_TANH) {
             if (ak == ANGK_Z1) angular_group<R, MODE, CUTK_TANH, ANGK_Z1>(S, X, W, acc, cgx, cgy, cgz, cap);
             else if (ak == ANGK_P4) angular_group<R, MODE, CUTK_TANH, ANGK_P4>(S, X, W, acc, cgx, cgy, cgz, cap);

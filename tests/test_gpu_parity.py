@@ -306,3 +306,36 @@ def test_cfg1_subset_and_fd(dev):
             worst = max(worst, abs(fd - f[a, d]) / max(1e-3, abs(f[a, d])))
     _log(f"cfg1 FD worst relative deviation {worst:.2e}")
     assert worst < 1e-5
+
+
+@pytest.mark.parametrize("cutoff", ["tanh", "tanhu", "cos", "exp", "poly2"])
+@pytest.mark.parametrize("zetas,kind", [((1.0, 2.0), "G3"), ((2.0, 3.0), "G3"), ((2.5, 1.0), "G3"), ((1.0, 4.0), "G9")])
+def test_nnp_cutoff_and_zeta_paths_vs_oracle(dev, cutoff, zetas, kind):
+    """Every cutoff type and angular-term pattern the descriptor kernels specialise on
+    (zeta all 1 / in {1, 2, 4} / other integers / non-integer via pow; G3 and G9), FP64 and
+    mixed, through energy and -grad E against the oracle."""
+    from oracle.bridge import oracle_nnp
+    from pantea_amd.bench_models import make_nnp
+    from pantea_amd.descriptors import ACSF, G2, G3, G9, CutoffFunction, NeighborElements
+
+    cfn = CutoffFunction.from_type(cutoff, 9.0)
+    Gk = G3 if kind == "G3" else G9
+
+    def acsf_of(el):
+        radial = ((G2(cfn, 0.0, 0.02), NeighborElements("H")), (G2(cfn, 0.5, 0.1), NeighborElements("O")))
+        angular = tuple((Gk(cfn, eta=eta, zeta=z, lambda0=lam, r_shift=0.0), NeighborElements(j, k))
+                        for (j, k) in (("H", "H"), ("H", "O"), ("O", "O"))
+                        for eta in (0.005, 0.03) for z in zetas for lam in (1.0, -1.0))
+        return ACSF(el, radial, angular)
+
+    s = _water("cfg0")
+    pot = make_nnp(acsf_of, (8, 8), seed=3, precision="float64")
+    pot.fit_scaler([s])
+    E, e_atom, f_self, f_full = oracle_nnp(pot, s).energy_forces(s.positions.cpu().numpy(), s.atom_types_host,
+                                                                 s.box_lengths)
+    e64, f64 = pot.energy_and_forces(s, mode="full", precision="float64")
+    assert abs(float(e64) - E) <= 1e-10 * np.sum(np.abs(e_atom))
+    assert _rel(f64.cpu().numpy(), f_full) < 1e-10
+    em, fm = pot.energy_and_forces(s, mode="full", precision="mixed")
+    assert abs(float(em) - E) <= 1e-4 * np.sum(np.abs(e_atom))
+    assert _rel(fm.cpu().numpy(), f_full) < 3e-3

@@ -230,8 +230,14 @@ __device__ __forceinline__ void angular_term(R c, R lam, R zeta, int izeta, R pr
 }
 
 // Angular-term evaluation specialised per group: ANGK_Z1 (every zeta = 1),
-// ANGK_P4 (every zeta in {1, 2, 4}) or the generic square-and-multiply / pow.
-enum { ANGK_Z1 = 0, ANGK_P4 = 1, ANGK_GEN = 2 };
+// ANGK_Z1N2 (every zeta = 1, at most two terms), ANGK_P4 (every zeta in
+// {1, 2, 4}) or the generic square-and-multiply / pow.  The specialised
+// forms leave the factor pre = 2^(1 - zeta) out of the pair loop: it scales
+// the group's sums (forward) or dE/dG (adjoint) once per centre.
+enum { ANGK_Z1 = 0, ANGK_P4 = 1, ANGK_GEN = 2, ANGK_Z1N2 = 3 };
+
+template <int ANG>
+__host__ __device__ constexpr int ang_terms() { return ANG == ANGK_Z1N2 ? 2 : PT_AG_ANG; }
 
 template <typename R> struct alignas(4 * sizeof(R)) V4 { R x, y, z, w; };
 
@@ -332,24 +338,27 @@ struct AngGroup {
     R rc, irc, cpre, tanh_pre;
     const BoxArg* box;
     R keta[PT_AG_ETA];     // -eta * kexp: exp(-eta S) = ex2k(keta S)
-    R lam[PT_AG_ANG], zeta[PT_AG_ANG], pre[PT_AG_ANG], dpre[PT_AG_ANG];
+    R lam[PT_AG_ANG], zeta[PT_AG_ANG], pre[PT_AG_ANG], zl[PT_AG_ANG];   // zl = zeta * lambda
     int izeta[PT_AG_ANG];
 };
 
 template <typename R, int ANG, bool DER>
 __device__ __forceinline__ void ang_eval(const AngGroup<R>& X, R cs, R (&v)[PT_AG_ANG], R (&dv)[PT_AG_ANG]) {
-    // terms q >= n_ang carry pre = dpre = lam = 0 and izeta = 1, so they are 0
+    // terms q >= n_ang carry pre = lam = zl = 0 and izeta = 1; in the
+    // specialised forms v = (1 + lam c)^zeta and dv = zeta lam (1 + lam c)^(zeta - 1)
+    // without pre, in the generic one both include pre
+    constexpr int NA = ang_terms<ANG>();
 #pragma unroll
-    for (int q = 0; q < PT_AG_ANG; ++q) {
-        if (ANG == ANGK_Z1) {
-            v[q] = X.pre[q] * (R(1) + X.lam[q] * cs);
-            dv[q] = DER ? X.dpre[q] : R(0);
+    for (int q = 0; q < NA; ++q) {
+        if (ANG == ANGK_Z1 || ANG == ANGK_Z1N2) {
+            v[q] = R(1) + X.lam[q] * cs;
+            dv[q] = DER ? X.zl[q] : R(0);
         } else if (ANG == ANGK_P4) {
             const R b = R(1) + X.lam[q] * cs;
             const R b2 = b * b;
             const int z = X.izeta[q];
-            v[q] = X.pre[q] * (z == 1 ? b : (z == 2 ? b2 : b2 * b2));
-            dv[q] = DER ? X.dpre[q] * (z == 1 ? R(1) : (z == 2 ? b : b * b2)) : R(0);
+            v[q] = z == 1 ? b : (z == 2 ? b2 : b2 * b2);
+            dv[q] = DER ? X.zl[q] * (z == 1 ? R(1) : (z == 2 ? b : b * b2)) : R(0);
         } else {
             angular_term<R>(cs, X.lam[q], X.zeta[q], X.izeta[q], X.pre[q], v[q], dv[q]);
         }
@@ -360,10 +369,17 @@ __device__ __forceinline__ void ang_eval(const AngGroup<R>& X, R cs, R (&v)[PT_A
 // per-wave queue of live pairs, and the pair body in full-wave batches.
 template <typename R, int MODE, int CUT, int ANG>
 __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<R>& X,
-                                              const R (&W)[PT_AG_ETA][PT_AG_ANG],
+                                              const R (&Win)[PT_AG_ETA][PT_AG_ANG],
                                               R (&acc)[PT_AG_ETA][PT_AG_ANG], R& cgx, R& cgy, R& cgz,
                                               int cap) {
     const int lane = wave_lane();
+    constexpr int NA = ang_terms<ANG>();
+    constexpr bool kPreOut = ANG != ANGK_GEN;   // pre applied outside the pair loop
+    R W[PT_AG_ETA][PT_AG_ANG];
+#pragma unroll
+    for (int e = 0; e < PT_AG_ETA; ++e)
+#pragma unroll
+        for (int q = 0; q < PT_AG_ANG; ++q) W[e][q] = kPreOut ? Win[e][q] * X.pre[q] : Win[e][q];
     // cutoff values of the group's neighbours
     for (int pass = 0; pass < (X.same ? 1 : 2); ++pass) {
         const int sb = pass == 0 ? X.bj : X.bk, sn = pass == 0 ? X.nj : X.nk;
@@ -399,7 +415,7 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
                 if (e < X.n_eta) {
                     const R rad = M<R>::ex2k(X.keta[e] * Ssum) * F;
 #pragma unroll
-                    for (int q = 0; q < PT_AG_ANG; ++q) acc[e][q] += rad * ang[q];
+                    for (int q = 0; q < NA; ++q) acc[e][q] += rad * ang[q];
                 }
             }
             return;
@@ -423,7 +439,7 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
                     const R ex = M<R>::ex2k(X.keta[e] * Ssum);
                     R U = R(0), V = R(0);
 #pragma unroll
-                    for (int q = 0; q < PT_AG_ANG; ++q) { U += W[e][q] * ang[q]; V += W[e][q] * dang[q]; }
+                    for (int q = 0; q < NA; ++q) { U += W[e][q] * ang[q]; V += W[e][q] * dang[q]; }
                     const R P = ex * U;
                     SP += P;
                     SKP += X.keta[e] * P;
@@ -466,7 +482,7 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
                     const R rdot = ex * (X0 + R(2.0 / M<R>::kexp) * X.keta[e] * X1);
                     const R radc = ex * F * cdot;
 #pragma unroll
-                    for (int q = 0; q < PT_AG_ANG; ++q) acc[e][q] += rdot * ang[q] + radc * dang[q];
+                    for (int q = 0; q < NA; ++q) acc[e][q] += rdot * ang[q] + radc * dang[q];
                 }
             }
         }
@@ -537,6 +553,12 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
             if (lane < take) body((int)(e & 0xffffu), (int)(e >> 16));
         }
         if (last && qn == 0) break;
+    }
+    if (kPreOut && MODE != MODE_ADJ) {
+#pragma unroll
+        for (int e = 0; e < PT_AG_ETA; ++e)
+#pragma unroll
+            for (int q = 0; q < PT_AG_ANG; ++q) acc[e][q] *= X.pre[q];
     }
     wave_sync();
 }
@@ -714,24 +736,26 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
             X.zeta[q] = on ? (R)G->zeta[q] : R(1);
             X.pre[q] = on ? (R)G->pre[q] : R(0);
             X.izeta[q] = on ? uniform(G->izeta[q]) : 1;
-            X.dpre[q] = on ? (R)(G->pre[q] * G->zeta[q] * G->lam[q]) : R(0);
+            X.zl[q] = on ? (R)(G->zeta[q] * G->lam[q]) : R(0);
             z1 = z1 && X.izeta[q] == 1;
             p4 = p4 && (X.izeta[q] == 1 || X.izeta[q] == 2 || X.izeta[q] == 4);
         }
         const int ck = (X.cut == PT_CUT_TANH || X.cut == PT_CUT_TANHU) ? CUTK_TANH
                        : (X.cut == PT_CUT_COS ? CUTK_COS : CUTK_GEN);
-        const int ak = z1 ? ANGK_Z1 : (p4 ? ANGK_P4 : ANGK_GEN);
+        const int ak = z1 ? (X.n_ang <= 2 ? ANGK_Z1N2 : ANGK_Z1) : (p4 ? ANGK_P4 : ANGK_GEN);
         wave_sync();
         if (MODE == MODE_TAN || (sizeof(R) == 8 && ck != CUTK_TANH)) {
             angular_group<R, MODE, CUTK_GEN, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
         } else if (sizeof(R) == 8) {
             angular_group<R, MODE, CUTK_TANH, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
         } else if (ck == CUTK_TANH) {
-            if (ak == ANGK_Z1) angular_group<R, MODE, CUTK_TANH, ANGK_Z1>(S, X, W, acc, cgx, cgy, cgz, cap);
+            if (ak == ANGK_Z1N2) angular_group<R, MODE, CUTK_TANH, ANGK_Z1N2>(S, X, W, acc, cgx, cgy, cgz, cap);
+            else if (ak == ANGK_Z1) angular_group<R, MODE, CUTK_TANH, ANGK_Z1>(S, X, W, acc, cgx, cgy, cgz, cap);
             else if (ak == ANGK_P4) angular_group<R, MODE, CUTK_TANH, ANGK_P4>(S, X, W, acc, cgx, cgy, cgz, cap);
             else angular_group<R, MODE, CUTK_TANH, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
         } else if (ck == CUTK_COS) {
-            if (ak == ANGK_Z1) angular_group<R, MODE, CUTK_COS, ANGK_Z1>(S, X, W, acc, cgx, cgy, cgz, cap);
+            if (ak == ANGK_Z1N2) angular_group<R, MODE, CUTK_COS, ANGK_Z1N2>(S, X, W, acc, cgx, cgy, cgz, cap);
+            else if (ak == ANGK_Z1) angular_group<R, MODE, CUTK_COS, ANGK_Z1>(S, X, W, acc, cgx, cgy, cgz, cap);
             else if (ak == ANGK_P4) angular_group<R, MODE, CUTK_COS, ANGK_P4>(S, X, W, acc, cgx, cgy, cgz, cap);
             else angular_group<R, MODE, CUTK_COS, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
         } else {

@@ -280,6 +280,14 @@ def main() -> int:
         fp64_value = n * args.fp64_steps / dt64
         _log(f"[bench] teacher FP64 MD: {1e3 * dt64 / args.fp64_steps:.3f} ms/step, {fp64_value:.4g} atom-steps/s")
 
+    # the student alone (no judge): the rate the surrogate would reach if every judge passed
+    student_value = None
+    if student is not None:
+        dt_st, _, _ = run_md(student, s, "mixed", args.teacher_steps, args.warmup, args)
+        student_value = n * args.teacher_steps / dt_st
+        _log(f"[bench] student mixed MD (no judge): {1e3 * dt_st / args.teacher_steps:.3f} ms/step, "
+             f"{student_value:.4g} atom-steps/s")
+
     # headline leg
     sur = None
     if student is not None:
@@ -369,6 +377,8 @@ def main() -> int:
                    "atoms": n, "skin_bohr": args.skin, "parallelism": "single GPU"},
         "fp64_atom_steps_per_s": fp64_value,
         "mixed_teacher_atom_steps_per_s": mixed_value,
+        "student_mixed_atom_steps_per_s": student_value,
+        "student_over_fp64": (student_value / fp64_value) if (student_value and fp64_value) else None,
         "value_over_fp64": (value / fp64_value) if fp64_value else None,
         "mixed_teacher_over_fp64": (mixed_value / fp64_value) if fp64_value else None,
         "surrogate": sur,

@@ -747,7 +747,8 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
         if (MODE == MODE_TAN || (sizeof(R) == 8 && ck != CUTK_TANH)) {
             angular_group<R, MODE, CUTK_GEN, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
         } else if (sizeof(R) == 8) {
-            angular_group<R, MODE, CUTK_TANH, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
+            if (ak == ANGK_GEN) angular_group<R, MODE, CUTK_TANH, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
+            else angular_group<R, MODE, CUTK_TANH, ANGK_P4>(S, X, W, acc, cgx, cgy, cgz, cap);
         } else if (ck == CUTK_TANH) {
             if (ak == ANGK_Z1N2) angular_group<R, MODE, CUTK_TANH, ANGK_Z1N2>(S, X, W, acc, cgx, cgy, cgz, cap);
             else if (ak == ANGK_Z1) angular_group<R, MODE, CUTK_TANH, ANGK_Z1>(S, X, W, acc, cgx, cgy, cgz, cap);

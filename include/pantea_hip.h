@@ -43,6 +43,17 @@ enum {
 /* Arithmetic of the descriptor kernels. */
 enum { PT_F64 = 0, PT_F32 = 1 };
 
+/* Descriptor kernel variants.  The precision argument of pt_acsf_forward /
+ * pt_acsf_adjoint may carry (kind << PT_ACSF_KIND_SHIFT), kind from
+ * pt_sf_table_kind() of the table's host copy: the launch then uses a kernel
+ * compiled for that table class only (fewer registers, higher occupancy).
+ * Results are those of PT_ACSF_GENERIC; a kernel that meets a group outside
+ * its class writes NaN for that centre instead of a wrong number. */
+#define PT_ACSF_KIND_SHIFT 8
+enum { PT_ACSF_GENERIC = 0,     /* any table */
+       PT_ACSF_TANH_P4 = 1,     /* every angular group: tanh/tanhu cutoff, zeta in {1, 2, 4} */
+       PT_ACSF_TANH_Z1N2 = 2 }; /* every angular group: tanh/tanhu cutoff, zeta = 1, <= 2 terms */
+
 /* Arithmetic of the MLP kernels (matrix cores). */
 enum {
     PT_MLP_F64 = 0,    /* v_mfma_f64_16x16x4_f64 */
@@ -149,6 +160,9 @@ int pt_nbr_build(int n, const double *pos, int pos_stride, const int *types, int
  * surrogate MD driver); no counterpart in the reference. */
 int pt_nbr_mark_rows(int n_sel, const int *sel, const long long *row_off, const int *nbr,
                      unsigned char *flags, pt_stream_t stream);
+
+/* Table class of a host-side table (host code, no device access). */
+int pt_sf_table_kind(const pt_sf_table *host_table);
 
 /* ------------------------------------------------------------------------
  * (b) ACSF forward: descriptor rows of the centres

@@ -24,6 +24,8 @@ PT_ERR_UNSUPPORTED = -4
 
 PT_F64 = 0
 PT_F32 = 1
+PT_ACSF_KIND_SHIFT = 8
+PT_ACSF_GENERIC, PT_ACSF_TANH_P4, PT_ACSF_TANH_Z1N2 = 0, 1, 2
 
 PT_MLP_F64 = 0
 PT_MLP_BF16 = 1
@@ -96,6 +98,7 @@ _SIGS = {
     "pt_nbr_build": (_c_int, [_c_int, _vp, _c_int, _vp, _c_int, _vp, _c_dbl, _vp, _vp, _vp, _vp, _vp,
                               _c_ll, ctypes.POINTER(_c_ll), ctypes.POINTER(_c_int), _vp, _c_size, _vp]),
     "pt_nbr_mark_rows": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
+    "pt_sf_table_kind": (_c_int, [ctypes.POINTER(SFTable)]),
     "pt_acsf_forward": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp,
                                  _vp, _c_int, _vp]),
     "pt_acsf_adjoint": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp,

@@ -251,14 +251,14 @@ struct Stage {
     V4<R>* Q;      // (r, r^2, fc, dfc/dr) -- cutoff of the current group
     R* D;          // [3][cap] TAN: delta_i - delta_j
     double* DA;    // [3][cap] ADJ: per-edge accumulators (FP64: fast LDS atomics)
-    int* el; int* slot;
+    int* slot;     // ADJ: row slot of each staged neighbour
     unsigned* queue;
 };
 
 template <typename R, int MODE>
 __host__ __device__ constexpr size_t stage_bytes_per_nbr() {
     return sizeof(double) * (MODE == MODE_ADJ ? 3 : 0) + sizeof(R) * (8 + (MODE == MODE_TAN ? 3 : 0)) +
-           2 * sizeof(int);
+           (MODE == MODE_ADJ ? sizeof(int) : 0);
 }
 
 // cap is a multiple of 4; the per-wave area is a multiple of 32 bytes
@@ -275,8 +275,7 @@ __device__ __forceinline__ Stage<R> carve(char* base, int cap) {
     s.P = reinterpret_cast<V4<R>*>(p); p += cap * sizeof(V4<R>);
     s.Q = reinterpret_cast<V4<R>*>(p); p += cap * sizeof(V4<R>);
     if (MODE == MODE_TAN) { s.D = reinterpret_cast<R*>(p); p += 3 * cap * sizeof(R); } else { s.D = nullptr; }
-    s.el = reinterpret_cast<int*>(p); p += cap * sizeof(int);
-    s.slot = reinterpret_cast<int*>(p); p += cap * sizeof(int);
+    if (MODE == MODE_ADJ) { s.slot = reinterpret_cast<int*>(p); p += cap * sizeof(int); } else { s.slot = nullptr; }
     s.queue = reinterpret_cast<unsigned*>(p);
     return s;
 }
@@ -299,15 +298,15 @@ __device__ __forceinline__ void mic3(const BoxArg& b, R& x, R& y, R& z) {
     z = mic_r(z, (R)b.L[2], (R)b.half[2]);
 }
 
-// Number of staged neighbours with type < t (rows are type-sorted).
-template <typename R>
-__device__ __forceinline__ int count_below(const Stage<R>& S, int m, int t) {
-    int cnt = 0;
-    for (int s0 = 0; s0 < m; s0 += kWave) {
-        const int s = s0 + wave_lane();
-        cnt += __popcll(__ballot(s < m && S.el[s] < t));
-    }
-    return uniform(cnt);
+// Staged neighbours with type < t for t = 0 .. kSegTypes (rows are type-sorted,
+// so each element is one contiguous segment).  Types are 1-based.
+constexpr int kSegTypes = 9;
+
+__device__ __forceinline__ int seg_below(const int (&seg)[kSegTypes + 1], int t) {
+    int v = seg[kSegTypes];
+#pragma unroll
+    for (int q = kSegTypes - 1; q >= 0; --q) v = t <= q ? seg[q] : v;
+    return v;
 }
 
 struct KArgs {
@@ -563,7 +562,7 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
     wave_sync();
 }
 
-template <typename R, int MODE>
+template <typename R, int MODE, int KIND>
 __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
     extern __shared__ __align__(32) char smem[];
     if (sizeof(R) == 8) exp2_tab_load();
@@ -592,6 +591,9 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
 
     // ---- stage the row ----------------------------------------------------
     int m = 0;
+    int seg[kSegTypes + 1];
+#pragma unroll
+    for (int q = 0; q <= kSegTypes; ++q) seg[q] = 0;
     for (int t0 = 0; t0 < len; t0 += kWave) {
         const int t = t0 + lane;
         bool keep = false;
@@ -613,8 +615,7 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
             qv.x = (R)r; qv.y = (R)r * (R)r; qv.z = R(0); qv.w = R(0);
             S.P[s] = pv;
             S.Q[s] = qv;
-            S.el[s] = (int)tj;
-            S.slot[s] = t;
+            if (MODE == MODE_ADJ) S.slot[s] = t;
             if (MODE == MODE_ADJ) { S.DA[s] = 0.0; S.DA[cap + s] = 0.0; S.DA[2 * cap + s] = 0.0; }
             if (MODE == MODE_TAN) {
                 double ex = dix, ey = diy, ez = diz;
@@ -628,8 +629,13 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
             eg[0] = R(0); eg[1] = R(0); eg[2] = R(0);
         }
         m += __popcll(bal);
+        const int tk = (int)tj;
+#pragma unroll
+        for (int q = 1; q <= kSegTypes; ++q) seg[q] += __popcll(__ballot(keep && tk < q));
     }
     m = uniform(m);
+#pragma unroll
+    for (int q = 0; q <= kSegTypes; ++q) seg[q] = uniform(seg[q]);
     wave_sync();
 
     R* out = (R*)a.out;
@@ -641,7 +647,7 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
     for (int g = 0; g < n_rg; ++g) {
         const pt_radial_group* G = &tab->rg[g];
         const int el = uniform(G->elem);
-        const int b = count_below(S, m, el), e = count_below(S, m, el + 1);
+        const int b = seg_below(seg, el), e = seg_below(seg, el + 1);
         const int nfun = uniform(G->n);
         const int cut = uniform(G->cut);
         const R rc = (R)G->rc, irc = (R)(1.0 / G->rc);
@@ -696,14 +702,15 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
 
     // ---- angular groups ----------------------------------------------------
     const int n_ag = uniform(tab->n_ag);
+    bool bad = false;   // a group outside this kernel's table class
     for (int g = 0; g < n_ag; ++g) {
         const pt_angular_group* G = &tab->ag[g];
         AngGroup<R> X;
         const int ej = uniform(G->ej), ek = uniform(G->ek);
-        X.bj = count_below(S, m, ej);
-        X.nj = count_below(S, m, ej + 1) - X.bj;
-        X.bk = count_below(S, m, ek);
-        X.nk = count_below(S, m, ek + 1) - X.bk;
+        X.bj = seg_below(seg, ej);
+        X.nj = seg_below(seg, ej + 1) - X.bj;
+        X.bk = seg_below(seg, ek);
+        X.nk = seg_below(seg, ek + 1) - X.bk;
         X.same = ej == ek;
         X.g3 = uniform(G->kind) == 3;
         X.mic = mic_jk;
@@ -744,7 +751,17 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
                        : (X.cut == PT_CUT_COS ? CUTK_COS : CUTK_GEN);
         const int ak = z1 ? (X.n_ang <= 2 ? ANGK_Z1N2 : ANGK_Z1) : (p4 ? ANGK_P4 : ANGK_GEN);
         wave_sync();
-        if (MODE == MODE_TAN || (sizeof(R) == 8 && ck != CUTK_TANH)) {
+        if constexpr (KIND == PT_ACSF_TANH_P4) {
+            if (ck == CUTK_TANH && ak != ANGK_GEN)
+                angular_group<R, MODE, CUTK_TANH, ANGK_P4>(S, X, W, acc, cgx, cgy, cgz, cap);
+            else
+                bad = true;
+        } else if constexpr (KIND == PT_ACSF_TANH_Z1N2) {
+            if (ck == CUTK_TANH && ak == ANGK_Z1N2)
+                angular_group<R, MODE, CUTK_TANH, ANGK_Z1N2>(S, X, W, acc, cgx, cgy, cgz, cap);
+            else
+                bad = true;
+        } else if (MODE == MODE_TAN || (sizeof(R) == 8 && ck != CUTK_TANH)) {
             angular_group<R, MODE, CUTK_GEN, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
         } else if (sizeof(R) == 8) {
             if (ak == ANGK_GEN) angular_group<R, MODE, CUTK_TANH, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
@@ -781,28 +798,32 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
         wave_sync();
     }
 
+    const R nan = __builtin_nan("");
+    if (MODE != MODE_ADJ && bad) {
+        for (int q = lane; q < a.ld; q += kWave) out[(size_t)c * a.ld + q] = nan;
+    }
     if (MODE == MODE_ADJ) {
         if (a.edge_grad) {
             R* eg = (R*)a.edge_grad;
             for (int s = lane; s < m; s += kWave) {
                 const long long e = off + S.slot[s];
-                eg[3 * e + 0] = (R)S.DA[s];
-                eg[3 * e + 1] = (R)S.DA[cap + s];
-                eg[3 * e + 2] = (R)S.DA[2 * cap + s];
+                eg[3 * e + 0] = bad ? nan : (R)S.DA[s];
+                eg[3 * e + 1] = bad ? nan : (R)S.DA[cap + s];
+                eg[3 * e + 2] = bad ? nan : (R)S.DA[2 * cap + s];
             }
         }
         if (a.f_self) {
             const double sx = wave_sum((double)cgx), sy = wave_sum((double)cgy), sz = wave_sum((double)cgz);
             if (lane == 0) {
-                a.f_self[3 * c + 0] = -sx;
-                a.f_self[3 * c + 1] = -sy;
-                a.f_self[3 * c + 2] = -sz;
+                a.f_self[3 * c + 0] = bad ? (double)nan : -sx;
+                a.f_self[3 * c + 1] = bad ? (double)nan : -sy;
+                a.f_self[3 * c + 2] = bad ? (double)nan : -sz;
             }
         }
     }
 }
 
-template <typename R, int MODE>
+template <typename R, int MODE, int KIND>
 int launch(KArgs a, hipStream_t stream) {
     if (a.n_centers <= 0) return PT_OK;
     a.cap = (std::max(a.cap, 1) + 3) / 4 * 4;
@@ -816,23 +837,42 @@ int launch(KArgs a, hipStream_t stream) {
         set_error("acsf: neighbour row of %d entries exceeds the LDS budget", a.cap);
         return PT_ERR_UNSUPPORTED;
     }
-    const int wpb = (int)std::min<size_t>(4, lds_budget / per_wave);
+    // waves per workgroup: the most resident waves per CU under the LDS limit
+    // (ties go to the larger workgroup)
+    const size_t cu = std::max(lds_per_cu(), lds_budget);
+    int wpb = 1;
+    size_t best = 0;
+    for (int w = 1; w <= 4; w *= 2) {
+        if ((size_t)w * per_wave > lds_budget) break;
+        const size_t waves = (size_t)w * (cu / ((size_t)w * per_wave));
+        if (waves >= best) { best = waves; wpb = w; }
+    }
     const int blocks = (a.n_centers + wpb - 1) / wpb;
-    hipLaunchKernelGGL((k_acsf<R, MODE>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, stream, a);
+    hipLaunchKernelGGL((k_acsf<R, MODE, KIND>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, stream, a);
     return launch_check("k_acsf");
 }
 
-int run(int precision, int mode, KArgs& a, hipStream_t st) {
-    if (precision == PT_F64) {
-        if (mode == MODE_FWD) return launch<double, MODE_FWD>(a, st);
-        if (mode == MODE_ADJ) return launch<double, MODE_ADJ>(a, st);
-        return launch<double, MODE_TAN>(a, st);
+template <typename R>
+int run_kind(int mode, int kind, KArgs& a, hipStream_t st) {
+    if (mode == MODE_TAN) return launch<R, MODE_TAN, PT_ACSF_GENERIC>(a, st);
+    switch (kind) {
+    case PT_ACSF_GENERIC:
+        return mode == MODE_FWD ? launch<R, MODE_FWD, PT_ACSF_GENERIC>(a, st) : launch<R, MODE_ADJ, PT_ACSF_GENERIC>(a, st);
+    case PT_ACSF_TANH_P4:
+        return mode == MODE_FWD ? launch<R, MODE_FWD, PT_ACSF_TANH_P4>(a, st) : launch<R, MODE_ADJ, PT_ACSF_TANH_P4>(a, st);
+    case PT_ACSF_TANH_Z1N2:
+        return mode == MODE_FWD ? launch<R, MODE_FWD, PT_ACSF_TANH_Z1N2>(a, st)
+                                : launch<R, MODE_ADJ, PT_ACSF_TANH_Z1N2>(a, st);
     }
-    if (precision == PT_F32) {
-        if (mode == MODE_FWD) return launch<float, MODE_FWD>(a, st);
-        if (mode == MODE_ADJ) return launch<float, MODE_ADJ>(a, st);
-        return launch<float, MODE_TAN>(a, st);
-    }
+    set_error("acsf: unknown kernel kind %d", kind);
+    return PT_ERR_ARG;
+}
+
+int run(int precision_kind, int mode, KArgs& a, hipStream_t st) {
+    const int precision = precision_kind & ((1 << PT_ACSF_KIND_SHIFT) - 1);
+    const int kind = precision_kind >> PT_ACSF_KIND_SHIFT;
+    if (precision == PT_F64) return run_kind<double>(mode, kind, a, st);
+    if (precision == PT_F32) return run_kind<float>(mode, kind, a, st);
     set_error("acsf: unknown precision %d", precision);
     return PT_ERR_ARG;
 }
@@ -904,4 +944,22 @@ extern "C" int pt_acsf_tangent(int precision, const double* pos4, const double* 
     a.delta = delta;
     a.center_only = center_only;
     return run(precision, MODE_TAN, a, st);
+}
+
+extern "C" int pt_sf_table_kind(const pt_sf_table* t) {
+    if (!t) return PT_ACSF_GENERIC;
+    if (t->n_ag <= 0) return PT_ACSF_GENERIC;
+    bool p4 = true, z1n2 = true;
+    for (int g = 0; g < t->n_ag && g < PT_MAX_GROUPS; ++g) {
+        const pt_angular_group& G = t->ag[g];
+        const bool tanh = G.cut == PT_CUT_TANH || G.cut == PT_CUT_TANHU;
+        bool allz1 = true, allp4 = true;
+        for (int q = 0; q < G.n_ang; ++q) {
+            allz1 = allz1 && G.izeta[q] == 1;
+            allp4 = allp4 && (G.izeta[q] == 1 || G.izeta[q] == 2 || G.izeta[q] == 4);
+        }
+        p4 = p4 && tanh && allp4;
+        z1n2 = z1n2 && tanh && allz1 && G.n_ang <= 2;
+    }
+    return z1n2 ? PT_ACSF_TANH_Z1N2 : (p4 ? PT_ACSF_TANH_P4 : PT_ACSF_GENERIC);
 }

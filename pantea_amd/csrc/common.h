@@ -36,6 +36,8 @@ void set_error(const char* fmt, ...);
 
 // Largest dynamic LDS allocation of one workgroup on the current device.
 size_t lds_limit();
+// LDS of one compute unit on the current device (all resident workgroups).
+size_t lds_per_cu();
 
 inline int launch_check(const char* what) {
     hipError_t e = hipGetLastError();

@@ -35,6 +35,22 @@ size_t lds_limit() {
     return cached;
 }
 
+size_t lds_per_cu() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 64 * 1024;
+    static thread_local int cached_dev = -1;
+    static thread_local size_t cached = 0;
+    if (dev != cached_dev) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+            v <= 0)
+            v = (int)lds_limit();
+        cached = (size_t)v;
+        cached_dev = dev;
+    }
+    return cached;
+}
+
 namespace {
 
 constexpr int kRedThreads = 256;

@@ -261,6 +261,8 @@ class DeviceTable:
         self.host = compile_table(rows)
         raw = bytes(self.host)
         self.buf = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
+        # kernel variant for this table class (pt_sf_table_kind of the same bytes)
+        self.kind = int(_lib.load().pt_sf_table_kind(ctypes.byref(self.host)))
 
     @property
     def ptr(self) -> int:
@@ -287,7 +289,7 @@ def acsf_forward(nl: NeighborList, table: DeviceTable, precision: int, n_centers
         out = torch.zeros((n_centers, ld), dtype=_dtype(precision), device=nl.pos4.device)
     box_p, _keep = _box_arg(nl.box_lengths)
     _lib.check(_lib.load().pt_acsf_forward(
-        precision, nl.pos4.data_ptr(), box_p, nl.row_off.data_ptr(), nl.nbr.data_ptr(), nl.max_row,
+        precision | (table.kind << _lib.PT_ACSF_KIND_SHIFT), nl.pos4.data_ptr(), box_p, nl.row_off.data_ptr(), nl.nbr.data_ptr(), nl.max_row,
         n_centers, center_begin, _vp(centers), table.ptr, out.data_ptr(), ld, _lib.stream_ptr()),
         "pt_acsf_forward")
     return out
@@ -297,7 +299,7 @@ def acsf_adjoint(nl: NeighborList, table: DeviceTable, precision: int, n_centers
                  center_begin: int = 0, centers=None, f_self=None, edge_grad=None):
     box_p, _keep = _box_arg(nl.box_lengths)
     _lib.check(_lib.load().pt_acsf_adjoint(
-        precision, nl.pos4.data_ptr(), box_p, nl.row_off.data_ptr(), nl.nbr.data_ptr(), nl.max_row,
+        precision | (table.kind << _lib.PT_ACSF_KIND_SHIFT), nl.pos4.data_ptr(), box_p, nl.row_off.data_ptr(), nl.nbr.data_ptr(), nl.max_row,
         n_centers, center_begin, _vp(centers), table.ptr, w.data_ptr(), ldw, _vp(f_self),
         _vp(edge_grad), _lib.stream_ptr()), "pt_acsf_adjoint")
 

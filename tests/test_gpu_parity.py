@@ -339,3 +339,28 @@ def test_nnp_cutoff_and_zeta_paths_vs_oracle(dev, cutoff, zetas, kind):
     em, fm = pot.energy_and_forces(s, mode="full", precision="mixed")
     assert abs(float(em) - E) <= 1e-4 * np.sum(np.abs(e_atom))
     assert _rel(fm.cpu().numpy(), f_full) < 3e-3
+
+
+def test_wrong_kernel_kind_fails_loudly(dev):
+    """A table launched with a kernel class that does not cover it yields NaN, not a wrong number."""
+    from pantea_amd import _lib
+    from pantea_amd.bench_models import teacher_acsf
+    from pantea_amd.engine import NeighborList, acsf_forward
+
+    s = _water("cfg0")
+    acsf = teacher_acsf("O")
+    tab = acsf.device_table(s.element_map, dev)
+    assert tab.kind == _lib.PT_ACSF_TANH_P4
+    nl = NeighborList(acsf.r_cutoff, s.element_map.n_types).build(s.positions, s.atom_types, s.box_lengths,
+                                                                   s.type_counts())
+    t_o = s.element_map.element_to_atom_type["O"]
+    b, cnt = nl.type_block(t_o)
+    good = acsf_forward(nl, tab, _lib.PT_F64, cnt, center_begin=b).cpu().numpy()
+    assert np.isfinite(good).all()
+    keep = tab.kind
+    try:
+        tab.kind = _lib.PT_ACSF_TANH_Z1N2   # this table has 4 terms with zeta = 4: not covered
+        bad = acsf_forward(nl, tab, _lib.PT_F64, cnt, center_begin=b).cpu().numpy()
+    finally:
+        tab.kind = keep
+    assert np.isnan(bad).all()

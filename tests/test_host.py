@@ -96,6 +96,33 @@ def test_table_compiler_splits_oversized_groups():
     assert sorted(seen) == list(range(30))
 
 
+def test_table_kind_classification():
+    """pt_sf_table_kind (host code) picks the specialised descriptor kernels only for tables they cover."""
+    import ctypes
+
+    from pantea_amd import _lib
+    from pantea_amd.bench_models import student_acsf, teacher_acsf
+    from pantea_amd.descriptors import ACSF, G2, G3, CutoffFunction, NeighborElements
+    from pantea_amd.engine import compile_table
+
+    lib = _lib.load()
+    emap = {"H": 1, "O": 2}
+
+    def kind(acsf):
+        return lib.pt_sf_table_kind(ctypes.byref(compile_table(acsf.rows(emap))))
+
+    assert kind(teacher_acsf("O")) == _lib.PT_ACSF_TANH_P4          # zeta in {1, 4}, tanh
+    assert kind(student_acsf("H")) == _lib.PT_ACSF_TANH_Z1N2        # zeta = 1, 2 terms, tanh
+    assert kind(teacher_acsf("O", cutoff="tanhu")) == _lib.PT_ACSF_TANH_P4
+    assert kind(teacher_acsf("O", cutoff="cos")) == _lib.PT_ACSF_GENERIC
+    cfn = CutoffFunction.from_type("tanh", 6.0)
+    z3 = ACSF("H", ((G2(cfn, 0.0, 0.1), NeighborElements("H")),),
+              ((G3(cfn, eta=0.1, zeta=3.0, lambda0=1.0), NeighborElements("H", "H")),))
+    assert kind(z3) == _lib.PT_ACSF_GENERIC
+    radial_only = ACSF("H", ((G2(cfn, 0.0, 0.1), NeighborElements("H")),), ())
+    assert kind(radial_only) == _lib.PT_ACSF_GENERIC
+
+
 def test_settings_from_json_fixture():
     from pantea_amd.potential import NNPSettings
 

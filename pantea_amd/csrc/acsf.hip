@@ -491,45 +491,41 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
     // distinct neighbour slots (few LDS atomic conflicts in the adjoint):
     //  - cross segments: lanes run over the shorter segment q, the other
     //    index is the diagonal k = (q + d) mod n_long, d = 0 .. n_long-1;
-    //  - one segment: k = (q + 1 + d) mod n for d < (n-1)/2, plus the
-    //    n/2 pairs (q, q + n/2) when n is even -- each unordered pair once.
+    //  - one segment: k = (q + 1 + d) mod n for d < n/2 -- each unordered
+    //    pair once (see below for the last diagonal of an even n).
     // Pair p = d * nrow + q; each lane advances p by 64 per step.
     const bool jrow = X.nj <= X.nk;
     const int nrow = X.same ? X.nj : (jrow ? X.nj : X.nk);
     const int ncol = X.same ? X.nj : (jrow ? X.nk : X.nj);
     const int brow = X.same ? X.bj : (jrow ? X.bj : X.bk);
     const int bcol = X.same ? X.bj : (jrow ? X.bk : X.bj);
-    const int h = X.same ? (X.nj - 1) / 2 : ncol;
-    const int p1 = nrow * h;
-    const int npairs = X.same ? p1 + ((X.nj % 2 == 0) ? X.nj / 2 : 0) : p1;
+    // one segment: h = n/2 diagonals d (k = q + 1 + d mod n); for even n the
+    // last diagonal pairs q with q + n/2 and holds each pair twice, so only
+    // q < n/2 is taken there
+    const int h = X.same ? X.nj / 2 : ncol;
+    const int npairs = nrow * h;
     if (npairs <= 0) return;
+    const int half_d = (X.same && X.nj % 2 == 0) ? h - 1 : -1;   // the doubled diagonal
+    const int half_q = X.nj / 2;
     const int d64 = uniform(kWave / nrow), r64 = uniform(kWave - d64 * nrow);
     int d = lane / nrow, q = lane - (lane / nrow) * nrow;
     // screen r_jk^2 < rc^2 (loosened by 1e-4 relative); the exact r_jk < rc
     // test is the cutoff's own (it is 0 beyond rc)
-    const R rc2 = X.rc * X.rc * R(1.0001);
+    const R rc2 = X.g3 ? X.rc * X.rc * R(1.0001) : R(INFINITY);
+    const int sk = X.same ? 1 : 0;
     int qn = 0, base = 0;
     while (true) {
         if (base < npairs) {
-            const int p = base + lane;
+            int k = q + sk + d;
+            k = k >= ncol ? k - ncol : k;
+            const int ia = brow + q, ib = bcol + k;
             bool live = false;
-            int ia = 0, ib = 0;
-            if (p < npairs) {
-                if (p < p1) {
-                    int k = X.same ? q + 1 + d : q + d;
-                    if (k >= ncol) k -= ncol;
-                    ia = brow + q;
-                    ib = bcol + k;
-                } else {
-                    const int t = p - p1;
-                    ia = X.bj + t;
-                    ib = X.bj + t + X.nj / 2;
-                }
+            if (base + lane < npairs && !(d == half_d && q >= half_q)) {
                 const V4<R> pa = S.P[ia], pb = S.P[ib];
                 R jx = pa.x - pb.x, jy = pa.y - pb.y, jz = pa.z - pb.z;
                 if (X.mic) mic3(*X.box, jx, jy, jz);
                 const R r2 = jx * jx + jy * jy + jz * jz;
-                live = (r2 > R(0)) && (!X.g3 || r2 < rc2);
+                live = (r2 > R(0)) & (r2 < rc2);
             }
             const unsigned long long bal = __ballot(live);
             if (live) S.queue[qn + lanes_below(bal)] = (unsigned)ia | ((unsigned)ib << 16);

@@ -686,12 +686,11 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
         }
         if (MODE != MODE_ADJ) {
 #pragma unroll
-            for (int q = 0; q < PT_RG_MAX; ++q) acc[q] = wave_sum(acc[q]);
-            if (lane == 0) {
-#pragma unroll
-                for (int q = 0; q < PT_RG_MAX; ++q)
-                    if (q < nfun) out[(size_t)c * a.ld + G->col[q]] = acc[q];
-            }
+            for (int q = 0; q < PT_RG_MAX; ++q)
+                if (q < nfun) {
+                    const R v = wave_sum(acc[q]);
+                    if (lane == 0) out[(size_t)c * a.ld + G->col[q]] = v;
+                }
         }
         wave_sync();
     }
@@ -777,19 +776,16 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
         }
 
         if (MODE != MODE_ADJ) {
+            // reduce only the group's own cells (uniform branches keep all lanes active)
 #pragma unroll
             for (int e = 0; e < PT_AG_ETA; ++e)
 #pragma unroll
-                for (int q = 0; q < PT_AG_ANG; ++q) acc[e][q] = wave_sum(acc[e][q]);
-            if (lane == 0) {
-#pragma unroll
-                for (int e = 0; e < PT_AG_ETA; ++e)
-#pragma unroll
-                    for (int q = 0; q < PT_AG_ANG; ++q) {
+                for (int q = 0; q < PT_AG_ANG; ++q)
+                    if (e < X.n_eta && q < X.n_ang) {
+                        const R v = wave_sum(acc[e][q]);
                         const int col = G->col[e * PT_AG_ANG + q];
-                        if (e < X.n_eta && q < X.n_ang && col >= 0) out[(size_t)c * a.ld + col] = acc[e][q];
+                        if (lane == 0 && col >= 0) out[(size_t)c * a.ld + col] = v;
                     }
-            }
         }
         wave_sync();
     }

@@ -253,6 +253,7 @@ struct Stage {
     double* DA;    // [3][cap] ADJ: per-edge accumulators (FP64: fast LDS atomics)
     int* slot;     // ADJ: row slot of each staged neighbour
     unsigned* queue;
+    R* W;          // ADJ, FP64: the group's dE/dG weights (read by broadcast)
 };
 
 template <typename R, int MODE>
@@ -264,7 +265,8 @@ __host__ __device__ constexpr size_t stage_bytes_per_nbr() {
 // cap is a multiple of 4; the per-wave area is a multiple of 32 bytes
 template <typename R, int MODE>
 __host__ __device__ constexpr size_t stage_bytes(int cap) {
-    return ((size_t)cap * stage_bytes_per_nbr<R, MODE>() + kQueue * sizeof(unsigned) + 31) / 32 * 32;
+    return ((size_t)cap * stage_bytes_per_nbr<R, MODE>() + kQueue * sizeof(unsigned) +
+            (MODE == MODE_ADJ ? PT_AG_ETA * PT_AG_ANG * sizeof(R) : 0) + 31) / 32 * 32;
 }
 
 template <typename R, int MODE>
@@ -277,6 +279,8 @@ __device__ __forceinline__ Stage<R> carve(char* base, int cap) {
     if (MODE == MODE_TAN) { s.D = reinterpret_cast<R*>(p); p += 3 * cap * sizeof(R); } else { s.D = nullptr; }
     if (MODE == MODE_ADJ) { s.slot = reinterpret_cast<int*>(p); p += cap * sizeof(int); } else { s.slot = nullptr; }
     s.queue = reinterpret_cast<unsigned*>(p);
+    p += kQueue * sizeof(unsigned);
+    s.W = MODE == MODE_ADJ ? reinterpret_cast<R*>(p) : nullptr;
     return s;
 }
 
@@ -374,11 +378,22 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
     const int lane = wave_lane();
     constexpr int NA = ang_terms<ANG>();
     constexpr bool kPreOut = ANG != ANGK_GEN;   // pre applied outside the pair loop
+    // FP64 adjoint: the weights live in LDS (broadcast reads), which frees
+    // 32 VGPRs in the pair loop
+    constexpr bool kWLds = MODE == MODE_ADJ && sizeof(R) == 8;
     R W[PT_AG_ETA][PT_AG_ANG];
 #pragma unroll
     for (int e = 0; e < PT_AG_ETA; ++e)
 #pragma unroll
         for (int q = 0; q < PT_AG_ANG; ++q) W[e][q] = kPreOut ? Win[e][q] * X.pre[q] : Win[e][q];
+    if (kWLds) {
+        if (lane == 0) {
+#pragma unroll
+            for (int e = 0; e < PT_AG_ETA; ++e)
+#pragma unroll
+                for (int q = 0; q < PT_AG_ANG; ++q) S.W[e * PT_AG_ANG + q] = W[e][q];
+        }
+    }
     // cutoff values of the group's neighbours
     for (int pass = 0; pass < (X.same ? 1 : 2); ++pass) {
         const int sb = pass == 0 ? X.bj : X.bk, sn = pass == 0 ? X.nj : X.nk;
@@ -438,7 +453,11 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
                     const R ex = M<R>::ex2k(X.keta[e] * Ssum);
                     R U = R(0), V = R(0);
 #pragma unroll
-                    for (int q = 0; q < NA; ++q) { U += W[e][q] * ang[q]; V += W[e][q] * dang[q]; }
+                    for (int q = 0; q < NA; ++q) {
+                        const R wq = kWLds ? S.W[e * PT_AG_ANG + q] : W[e][q];
+                        U += wq * ang[q];
+                        V += wq * dang[q];
+                    }
                     const R P = ex * U;
                     SP += P;
                     SKP += X.keta[e] * P;

@@ -240,6 +240,7 @@ template <int ANG>
 __host__ __device__ constexpr int ang_terms() { return ANG == ANGK_Z1N2 ? 2 : PT_AG_ANG; }
 
 template <typename R> struct alignas(4 * sizeof(R)) V4 { R x, y, z, w; };
+template <typename R> struct alignas(2 * sizeof(R)) V2 { R x, y; };
 
 // Live-pair queue of one wavefront: (a | b << 16) indices of pairs that
 // pass the r_jk screen.  Batches of 64 are taken as soon as they are full.
@@ -248,7 +249,8 @@ constexpr int kQueue = 128;
 template <typename R>
 struct Stage {
     V4<R>* P;      // (x, y, z, 1/r) of R_ij = r_i - r_j
-    V4<R>* Q;      // (r, r^2, fc, dfc/dr) -- cutoff of the current group
+    V4<R>* Q;      // ADJ, TAN: (r, r^2, fc, dfc/dr) -- cutoff of the current group
+    V2<R>* Q2;     // FWD: (r, fc)
     R* D;          // [3][cap] TAN: delta_i - delta_j
     double* DA;    // [3][cap] ADJ: per-edge accumulators (FP64: fast LDS atomics)
     int* slot;     // ADJ: row slot of each staged neighbour
@@ -258,8 +260,8 @@ struct Stage {
 
 template <typename R, int MODE>
 __host__ __device__ constexpr size_t stage_bytes_per_nbr() {
-    return sizeof(double) * (MODE == MODE_ADJ ? 3 : 0) + sizeof(R) * (8 + (MODE == MODE_TAN ? 3 : 0)) +
-           (MODE == MODE_ADJ ? sizeof(int) : 0);
+    return sizeof(double) * (MODE == MODE_ADJ ? 3 : 0) +
+           sizeof(R) * (MODE == MODE_FWD ? 6 : 8 + (MODE == MODE_TAN ? 3 : 0)) + (MODE == MODE_ADJ ? sizeof(int) : 0);
 }
 
 // cap is a multiple of 4; the per-wave area is a multiple of 32 bytes
@@ -275,7 +277,13 @@ __device__ __forceinline__ Stage<R> carve(char* base, int cap) {
     char* p = base;
     if (MODE == MODE_ADJ) { s.DA = reinterpret_cast<double*>(p); p += 3 * cap * sizeof(double); } else { s.DA = nullptr; }
     s.P = reinterpret_cast<V4<R>*>(p); p += cap * sizeof(V4<R>);
-    s.Q = reinterpret_cast<V4<R>*>(p); p += cap * sizeof(V4<R>);
+    if (MODE == MODE_FWD) {
+        s.Q = nullptr;
+        s.Q2 = reinterpret_cast<V2<R>*>(p); p += cap * sizeof(V2<R>);
+    } else {
+        s.Q2 = nullptr;
+        s.Q = reinterpret_cast<V4<R>*>(p); p += cap * sizeof(V4<R>);
+    }
     if (MODE == MODE_TAN) { s.D = reinterpret_cast<R*>(p); p += 3 * cap * sizeof(R); } else { s.D = nullptr; }
     if (MODE == MODE_ADJ) { s.slot = reinterpret_cast<int*>(p); p += cap * sizeof(int); } else { s.slot = nullptr; }
     s.queue = reinterpret_cast<unsigned*>(p);
@@ -306,11 +314,9 @@ __device__ __forceinline__ void mic3(const BoxArg& b, R& x, R& y, R& z) {
 // so each element is one contiguous segment).  Types are 1-based.
 constexpr int kSegTypes = 9;
 
-__device__ __forceinline__ int seg_below(const int (&seg)[kSegTypes + 1], int t) {
-    int v = seg[kSegTypes];
-#pragma unroll
-    for (int q = kSegTypes - 1; q >= 0; --q) v = t <= q ? seg[q] : v;
-    return v;
+// the counts live in lanes 0 .. kSegTypes of one VGPR (no private array)
+__device__ __forceinline__ int seg_below(int segv, int t) {
+    return __builtin_amdgcn_readlane(segv, t < 0 ? 0 : (t > kSegTypes ? kSegTypes : t));
 }
 
 struct KArgs {
@@ -399,9 +405,14 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
         const int sb = pass == 0 ? X.bj : X.bk, sn = pass == 0 ? X.nj : X.nk;
         for (int s = sb + lane; s < sb + sn; s += kWave) {
             R f, df;
-            cutoff_k<R, CUT>(X.cut, X.rc, X.irc, X.cpre, X.tanh_pre, S.Q[s].x, f, df);
-            S.Q[s].z = f;
-            S.Q[s].w = df;
+            if constexpr (MODE == MODE_FWD) {
+                cutoff_k<R, CUT>(X.cut, X.rc, X.irc, X.cpre, X.tanh_pre, S.Q2[s].x, f, df);
+                S.Q2[s].y = f;
+            } else {
+                cutoff_k<R, CUT>(X.cut, X.rc, X.irc, X.cpre, X.tanh_pre, S.Q[s].x, f, df);
+                S.Q[s].z = f;
+                S.Q[s].w = df;
+            }
         }
     }
     wave_sync();
@@ -409,7 +420,15 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
     // one live pair (ia, ib)
     auto body = [&](const int ia, const int ib) {
         const V4<R> pa = S.P[ia], pb = S.P[ib];
-        const V4<R> qa = S.Q[ia], qb = S.Q[ib];
+        V4<R> qa, qb;   // (r, r^2, fc, dfc)
+        if constexpr (MODE == MODE_FWD) {
+            const V2<R> ua = S.Q2[ia], ub = S.Q2[ib];
+            qa.x = ua.x; qa.y = ua.x * ua.x; qa.z = ua.y; qa.w = R(0);
+            qb.x = ub.x; qb.y = ub.x * ub.x; qb.z = ub.y; qb.w = R(0);
+        } else {
+            qa = S.Q[ia];
+            qb = S.Q[ib];
+        }
         R jx = pa.x - pb.x, jy = pa.y - pb.y, jz = pa.z - pb.z;
         if (X.mic) mic3(*X.box, jx, jy, jz);
         const R rjk2 = jx * jx + jy * jy + jz * jz;   // > 0 (screened)
@@ -578,8 +597,7 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
 }
 
 template <typename R, int MODE, int KIND>
-__global__ __launch_bounds__(256) void k_acsf(KArgs a) {
-    extern __shared__ __align__(32) char smem[];
+__device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
     if (sizeof(R) == 8) exp2_tab_load();
     const int lane = wave_lane();
     const int wv = threadIdx.x >> 6;
@@ -606,9 +624,7 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
 
     // ---- stage the row ----------------------------------------------------
     int m = 0;
-    int seg[kSegTypes + 1];
-#pragma unroll
-    for (int q = 0; q <= kSegTypes; ++q) seg[q] = 0;
+    int seg = 0;   // lane q: staged neighbours with type < q
     for (int t0 = 0; t0 < len; t0 += kWave) {
         const int t = t0 + lane;
         bool keep = false;
@@ -629,7 +645,13 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
             pv.x = (R)dx; pv.y = (R)dy; pv.z = (R)dz; pv.w = (R)(1.0 / r);
             qv.x = (R)r; qv.y = (R)r * (R)r; qv.z = R(0); qv.w = R(0);
             S.P[s] = pv;
-            S.Q[s] = qv;
+            if constexpr (MODE == MODE_FWD) {
+                V2<R> uv;
+                uv.x = qv.x; uv.y = R(0);
+                S.Q2[s] = uv;
+            } else {
+                S.Q[s] = qv;
+            }
             if (MODE == MODE_ADJ) S.slot[s] = t;
             if (MODE == MODE_ADJ) { S.DA[s] = 0.0; S.DA[cap + s] = 0.0; S.DA[2 * cap + s] = 0.0; }
             if (MODE == MODE_TAN) {
@@ -646,11 +668,12 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
         m += __popcll(bal);
         const int tk = (int)tj;
 #pragma unroll
-        for (int q = 1; q <= kSegTypes; ++q) seg[q] += __popcll(__ballot(keep && tk < q));
+        for (int q = 1; q <= kSegTypes; ++q) {
+            const int cq = __popcll(__ballot(keep && tk < q));
+            seg += lane == q ? cq : 0;
+        }
     }
     m = uniform(m);
-#pragma unroll
-    for (int q = 0; q <= kSegTypes; ++q) seg[q] = uniform(seg[q]);
     wave_sync();
 
     R* out = (R*)a.out;
@@ -676,8 +699,7 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
             wgt[q] = (MODE == MODE_ADJ && q < nfun) ? w[(size_t)c * a.ldw + G->col[q]] : R(0);
         }
         for (int s = b + lane; s < e; s += kWave) {
-            const V4<R> qv = S.Q[s];
-            const R r = qv.x;
+            const R r = (MODE == MODE_FWD) ? S.Q2[s].x : S.Q[s].x;
             R f, df;
             cutoff<R>(cut, rc, irc, tanh_pre, r, f, df);
             R rdot = R(0);
@@ -834,6 +856,21 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
     }
 }
 
+// Two entry kernels over the same body: the second asks the compiler for
+// four resident waves per SIMD (<= 128 VGPRs); the FP64 forward of the
+// specialised table classes fits that without spills.
+template <typename R, int MODE, int KIND>
+__global__ __launch_bounds__(256) void k_acsf(KArgs a) {
+    extern __shared__ __align__(32) char smem[];
+    acsf_centre<R, MODE, KIND>(a, smem);
+}
+
+template <typename R, int MODE, int KIND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_acsf_w4(KArgs a) {
+    extern __shared__ __align__(32) char smem[];
+    acsf_centre<R, MODE, KIND>(a, smem);
+}
+
 template <typename R, int MODE, int KIND>
 int launch(KArgs a, hipStream_t stream) {
     if (a.n_centers <= 0) return PT_OK;
@@ -859,7 +896,10 @@ int launch(KArgs a, hipStream_t stream) {
         if (waves >= best) { best = waves; wpb = w; }
     }
     const int blocks = (a.n_centers + wpb - 1) / wpb;
-    hipLaunchKernelGGL((k_acsf<R, MODE, KIND>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, stream, a);
+    if constexpr (sizeof(R) == 8 && MODE == MODE_FWD && KIND != PT_ACSF_GENERIC)
+        hipLaunchKernelGGL((k_acsf_w4<R, MODE, KIND>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, stream, a);
+    else
+        hipLaunchKernelGGL((k_acsf<R, MODE, KIND>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, stream, a);
     return launch_check("k_acsf");
 }
 

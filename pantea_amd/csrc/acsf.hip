@@ -251,6 +251,7 @@ struct Stage {
     V4<R>* P;      // (x, y, z, 1/r) of R_ij = r_i - r_j
     V4<R>* Q;      // ADJ, TAN: (r, r^2, fc, dfc/dr) -- cutoff of the current group
     V2<R>* Q2;     // FWD: (r, fc)
+    V2<R>* QF;     // compact FP64 ADJ: (fc, dfc); r is P.w
     R* D;          // [3][cap] TAN: delta_i - delta_j
     double* DA;    // [3][cap] ADJ: per-edge accumulators (FP64: fast LDS atomics)
     int* slot;     // ADJ: row slot of each staged neighbour
@@ -258,8 +259,15 @@ struct Stage {
     R* W;          // ADJ, FP64: the group's dE/dG weights (read by broadcast)
 };
 
+// FP64 adjoint: a compact record (x, y, z, r) + (fc, dfc) and no slot array
+// (1/r is recomputed per pair, edge positions by a second pass over the row),
+// which fits three waves per SIMD in LDS.
+template <typename R, int MODE>
+__host__ __device__ constexpr bool compact_adj() { return MODE == MODE_ADJ && sizeof(R) == 8; }
+
 template <typename R, int MODE>
 __host__ __device__ constexpr size_t stage_bytes_per_nbr() {
+    if (compact_adj<R, MODE>()) return sizeof(double) * 3 + sizeof(R) * 6;
     return sizeof(double) * (MODE == MODE_ADJ ? 3 : 0) +
            sizeof(R) * (MODE == MODE_FWD ? 6 : 8 + (MODE == MODE_TAN ? 3 : 0)) + (MODE == MODE_ADJ ? sizeof(int) : 0);
 }
@@ -277,15 +285,20 @@ __device__ __forceinline__ Stage<R> carve(char* base, int cap) {
     char* p = base;
     if (MODE == MODE_ADJ) { s.DA = reinterpret_cast<double*>(p); p += 3 * cap * sizeof(double); } else { s.DA = nullptr; }
     s.P = reinterpret_cast<V4<R>*>(p); p += cap * sizeof(V4<R>);
-    if (MODE == MODE_FWD) {
-        s.Q = nullptr;
+    s.Q = nullptr; s.Q2 = nullptr; s.QF = nullptr;
+    if (compact_adj<R, MODE>()) {
+        s.QF = reinterpret_cast<V2<R>*>(p); p += cap * sizeof(V2<R>);
+    } else if (MODE == MODE_FWD) {
         s.Q2 = reinterpret_cast<V2<R>*>(p); p += cap * sizeof(V2<R>);
     } else {
-        s.Q2 = nullptr;
         s.Q = reinterpret_cast<V4<R>*>(p); p += cap * sizeof(V4<R>);
     }
     if (MODE == MODE_TAN) { s.D = reinterpret_cast<R*>(p); p += 3 * cap * sizeof(R); } else { s.D = nullptr; }
-    if (MODE == MODE_ADJ) { s.slot = reinterpret_cast<int*>(p); p += cap * sizeof(int); } else { s.slot = nullptr; }
+    if (MODE == MODE_ADJ && !compact_adj<R, MODE>()) {
+        s.slot = reinterpret_cast<int*>(p); p += cap * sizeof(int);
+    } else {
+        s.slot = nullptr;
+    }
     s.queue = reinterpret_cast<unsigned*>(p);
     p += kQueue * sizeof(unsigned);
     s.W = MODE == MODE_ADJ ? reinterpret_cast<R*>(p) : nullptr;
@@ -405,7 +418,12 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
         const int sb = pass == 0 ? X.bj : X.bk, sn = pass == 0 ? X.nj : X.nk;
         for (int s = sb + lane; s < sb + sn; s += kWave) {
             R f, df;
-            if constexpr (MODE == MODE_FWD) {
+            if constexpr (compact_adj<R, MODE>()) {
+                cutoff_k<R, CUT>(X.cut, X.rc, X.irc, X.cpre, X.tanh_pre, S.P[s].w, f, df);
+                V2<R> fv;
+                fv.x = f; fv.y = df;
+                S.QF[s] = fv;
+            } else if constexpr (MODE == MODE_FWD) {
                 cutoff_k<R, CUT>(X.cut, X.rc, X.irc, X.cpre, X.tanh_pre, S.Q2[s].x, f, df);
                 S.Q2[s].y = f;
             } else {
@@ -421,13 +439,24 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
     auto body = [&](const int ia, const int ib) {
         const V4<R> pa = S.P[ia], pb = S.P[ib];
         V4<R> qa, qb;   // (r, r^2, fc, dfc)
-        if constexpr (MODE == MODE_FWD) {
+        R ira, irb;     // 1 / r
+        if constexpr (compact_adj<R, MODE>()) {
+            const V2<R> ua = S.QF[ia], ub = S.QF[ib];
+            qa.x = pa.w; qa.y = pa.w * pa.w; qa.z = ua.x; qa.w = ua.y;
+            qb.x = pb.w; qb.y = pb.w * pb.w; qb.z = ub.x; qb.w = ub.y;
+            ira = M<R>::rcp(pa.w);
+            irb = M<R>::rcp(pb.w);
+        } else if constexpr (MODE == MODE_FWD) {
             const V2<R> ua = S.Q2[ia], ub = S.Q2[ib];
             qa.x = ua.x; qa.y = ua.x * ua.x; qa.z = ua.y; qa.w = R(0);
             qb.x = ub.x; qb.y = ub.x * ub.x; qb.z = ub.y; qb.w = R(0);
         } else {
             qa = S.Q[ia];
             qb = S.Q[ib];
+        }
+        if constexpr (!compact_adj<R, MODE>()) {
+            ira = pa.w;
+            irb = pb.w;
         }
         R jx = pa.x - pb.x, jy = pa.y - pb.y, jz = pa.z - pb.z;
         if (X.mic) mic3(*X.box, jx, jy, jz);
@@ -438,7 +467,7 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
         if (X.g3) cutoff_k<R, CUT>(X.cut, X.rc, X.irc, X.cpre, X.tanh_pre, rjk, fjk, dfjk);
         const R Ssum = X.g3 ? qa.y + qb.y + rjk2 : qa.y + qb.y;
         const R F = qa.z * qb.z * fjk;
-        const R iab = pa.w * pb.w;
+        const R iab = ira * irb;
         const R cs = (pa.x * pb.x + pa.y * pb.y + pa.z * pb.z) * iab;
         R ang[PT_AG_ANG], dang[PT_AG_ANG];
         ang_eval<R, ANG, MODE != MODE_FWD>(X, cs, ang, dang);
@@ -455,7 +484,6 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
         }
         // derivative building blocks
         const R ra = qa.x, rb = qb.x;
-        const R ira = pa.w, irb = pb.w;
         const R ira2 = ira * ira, irb2 = irb * irb;
         const R cax = pb.x * iab - cs * pa.x * ira2, cay = pb.y * iab - cs * pa.y * ira2,
                 caz = pb.z * iab - cs * pa.z * ira2;  // d cos / d R_ij
@@ -642,17 +670,20 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
         if (keep) {
             const int s = m + lanes_below(bal);
             V4<R> pv, qv;
-            pv.x = (R)dx; pv.y = (R)dy; pv.z = (R)dz; pv.w = (R)(1.0 / r);
+            pv.x = (R)dx; pv.y = (R)dy; pv.z = (R)dz;
+            pv.w = compact_adj<R, MODE>() ? (R)r : (R)(1.0 / r);
             qv.x = (R)r; qv.y = (R)r * (R)r; qv.z = R(0); qv.w = R(0);
             S.P[s] = pv;
-            if constexpr (MODE == MODE_FWD) {
+            if constexpr (compact_adj<R, MODE>()) {
+                // (fc, dfc) are written per group
+            } else if constexpr (MODE == MODE_FWD) {
                 V2<R> uv;
                 uv.x = qv.x; uv.y = R(0);
                 S.Q2[s] = uv;
             } else {
                 S.Q[s] = qv;
             }
-            if (MODE == MODE_ADJ) S.slot[s] = t;
+            if (MODE == MODE_ADJ && !compact_adj<R, MODE>()) S.slot[s] = t;
             if (MODE == MODE_ADJ) { S.DA[s] = 0.0; S.DA[cap + s] = 0.0; S.DA[2 * cap + s] = 0.0; }
             if (MODE == MODE_TAN) {
                 double ex = dix, ey = diy, ez = diz;
@@ -699,7 +730,10 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
             wgt[q] = (MODE == MODE_ADJ && q < nfun) ? w[(size_t)c * a.ldw + G->col[q]] : R(0);
         }
         for (int s = b + lane; s < e; s += kWave) {
-            const R r = (MODE == MODE_FWD) ? S.Q2[s].x : S.Q[s].x;
+            R r;
+            if constexpr (compact_adj<R, MODE>()) r = S.P[s].w;
+            else if constexpr (MODE == MODE_FWD) r = S.Q2[s].x;
+            else r = S.Q[s].x;
             R f, df;
             cutoff<R>(cut, rc, irc, tanh_pre, r, f, df);
             R rdot = R(0);
@@ -719,7 +753,7 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
                 }
             }
             if (MODE == MODE_ADJ) {
-                const R sc = dsum * pv.w;
+                const R sc = dsum * (compact_adj<R, MODE>() ? M<R>::rcp(pv.w) : pv.w);
                 const R gx = sc * pv.x, gy = sc * pv.y, gz = sc * pv.z;
                 cgx += gx; cgy += gy; cgz += gz;
                 S.DA[s] -= (double)gx; S.DA[cap + s] -= (double)gy; S.DA[2 * cap + s] -= (double)gz;
@@ -838,11 +872,35 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
     if (MODE == MODE_ADJ) {
         if (a.edge_grad) {
             R* eg = (R*)a.edge_grad;
-            for (int s = lane; s < m; s += kWave) {
-                const long long e = off + S.slot[s];
-                eg[3 * e + 0] = bad ? nan : (R)S.DA[s];
-                eg[3 * e + 1] = bad ? nan : (R)S.DA[cap + s];
-                eg[3 * e + 2] = bad ? nan : (R)S.DA[2 * cap + s];
+            if constexpr (compact_adj<R, MODE>()) {
+                // the staged order again: same test as the staging pass
+                int ms = 0;
+                for (int t0 = 0; t0 < len; t0 += kWave) {
+                    const int t = t0 + lane;
+                    bool keep = false;
+                    if (t < len) {
+                        const double4 pj = reinterpret_cast<const double4*>(a.pos4)[a.nbr[off + t]];
+                        double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                        const double r = exact_dist(dx, dy, dz, a.box);
+                        keep = (r <= rcmax) && (r > 0.0);
+                    }
+                    const unsigned long long bal = __ballot(keep);
+                    if (keep) {
+                        const int s = ms + lanes_below(bal);
+                        const long long e = off + t;
+                        eg[3 * e + 0] = bad ? nan : (R)S.DA[s];
+                        eg[3 * e + 1] = bad ? nan : (R)S.DA[cap + s];
+                        eg[3 * e + 2] = bad ? nan : (R)S.DA[2 * cap + s];
+                    }
+                    ms += __popcll(bal);
+                }
+            } else {
+                for (int s = lane; s < m; s += kWave) {
+                    const long long e = off + S.slot[s];
+                    eg[3 * e + 0] = bad ? nan : (R)S.DA[s];
+                    eg[3 * e + 1] = bad ? nan : (R)S.DA[cap + s];
+                    eg[3 * e + 2] = bad ? nan : (R)S.DA[2 * cap + s];
+                }
             }
         }
         if (a.f_self) {

@@ -314,7 +314,11 @@ def main() -> int:
     launch_ops = (adj_ops if mode else fwd_ops) * n / 2.0
     achieved = launch_ops / (launch_ms * 1e-3) / 1e12 if launch_ms > 0 else 0.0
     peak = PEAK_FP64_TFLOPS if fp64 else PEAK_FP32_TFLOPS
-    kname = f"k_acsf<{'double' if fp64 else 'float'}, {mode}>"
+    # the launch's kernel: table class (pt_sf_table_kind) and the four-wave FP64 forward variant
+    pot_of = student if is_student else teacher
+    kind = pot_of.atomic_potentials["O"].descriptor.device_table(s.element_map, s.device).kind
+    w4 = "_w4" if (fp64 and mode == 0 and kind != 0) else ""
+    kname = f"k_acsf{w4}<{'double' if fp64 else 'float'}, {mode}, {kind}>"
     roofline = {
         "bound": "mfma",
         "achieved": achieved,

@@ -392,8 +392,7 @@ __device__ __forceinline__ void ang_eval(const AngGroup<R>& X, R cs, R (&v)[PT_A
 template <typename R, int MODE, int CUT, int ANG>
 __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<R>& X,
                                               const R (&Win)[PT_AG_ETA][PT_AG_ANG],
-                                              R (&acc)[PT_AG_ETA][PT_AG_ANG], R& cgx, R& cgy, R& cgz,
-                                              int cap) {
+                                              R (&acc)[PT_AG_ETA][PT_AG_ANG], int cap) {
     const int lane = wave_lane();
     constexpr int NA = ang_terms<ANG>();
     constexpr bool kPreOut = ANG != ANGK_GEN;   // pre applied outside the pair loop
@@ -485,10 +484,6 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
         // derivative building blocks
         const R ra = qa.x, rb = qb.x;
         const R ira2 = ira * ira, irb2 = irb * irb;
-        const R cax = pb.x * iab - cs * pa.x * ira2, cay = pb.y * iab - cs * pa.y * ira2,
-                caz = pb.z * iab - cs * pa.z * ira2;  // d cos / d R_ij
-        const R cbx = pa.x * iab - cs * pb.x * irb2, cby = pa.y * iab - cs * pb.y * irb2,
-                cbz = pa.z * iab - cs * pb.z * irb2;  // d cos / d R_ik
         if (!X.g3) irjk = R(0);
         // d F / d r_ij etc. (without the exp factor)
         const R Fa = qa.w * qb.z * fjk, Fb = qa.z * qb.w * fjk, Fj = qa.z * qb.z * dfjk;
@@ -518,19 +513,27 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
             const R d_rik = Fb * SP + rb * F2 * SKP;
             const R d_rjk = X.g3 ? (Fj * SP + rjk * F2 * SKP) : R(0);
             const R d_c = F * SV;
-            const R sa = d_rij * ira, sb = d_rik * irb, sj = d_rjk * irjk;
-            const R gax = sa * pa.x + d_c * cax + sj * jx;
-            const R gay = sa * pa.y + d_c * cay + sj * jy;
-            const R gaz = sa * pa.z + d_c * caz + sj * jz;
-            const R gbx = sb * pb.x + d_c * cbx - sj * jx;
-            const R gby = sb * pb.y + d_c * cby - sj * jy;
-            const R gbz = sb * pb.z + d_c * cbz - sj * jz;
-            cgx += gax + gbx; cgy += gay + gby; cgz += gaz + gbz;
+            // d cos / d R_ij = R_ik/(r_ij r_ik) - cos R_ij / r_ij^2 (and a <-> b), folded:
+            // g_a = (s_a - d_c cos / r_ij^2) R_ij + (d_c / (r_ij r_ik)) R_ik + s_j R_jk
+            const R sj = d_rjk * irjk;
+            const R al = d_rij * ira - d_c * cs * ira2;
+            const R bl = d_rik * irb - d_c * cs * irb2;
+            const R be = d_c * iab;
+            const R gax = al * pa.x + be * pb.x + sj * jx;
+            const R gay = al * pa.y + be * pb.y + sj * jy;
+            const R gaz = al * pa.z + be * pb.z + sj * jz;
+            const R gbx = bl * pb.x + be * pa.x - sj * jx;
+            const R gby = bl * pb.y + be * pa.y - sj * jy;
+            const R gbz = bl * pb.z + be * pa.z - sj * jz;
             lds_add(&S.DA[ia], -(double)gax); lds_add(&S.DA[cap + ia], -(double)gay);
             lds_add(&S.DA[2 * cap + ia], -(double)gaz);
             lds_add(&S.DA[ib], -(double)gbx); lds_add(&S.DA[cap + ib], -(double)gby);
             lds_add(&S.DA[2 * cap + ib], -(double)gbz);
         } else {  // MODE_TAN
+            const R cax = pb.x * iab - cs * pa.x * ira2, cay = pb.y * iab - cs * pa.y * ira2,
+                    caz = pb.z * iab - cs * pa.z * ira2;  // d cos / d R_ij
+            const R cbx = pa.x * iab - cs * pb.x * irb2, cby = pa.y * iab - cs * pb.y * irb2,
+                    cbz = pa.z * iab - cs * pb.z * irb2;  // d cos / d R_ik
             const R ax = S.D[ia], ay = S.D[cap + ia], az = S.D[2 * cap + ia];
             const R bx = S.D[ib], by = S.D[cap + ib], bz = S.D[2 * cap + ib];
             const R tij = (pa.x * ax + pa.y * ay + pa.z * az) * ira;
@@ -709,7 +712,6 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
 
     R* out = (R*)a.out;
     const R* w = (const R*)a.w;
-    R cgx = R(0), cgy = R(0), cgz = R(0);  // ADJ: d E_i / d r_i (per-lane part)
 
     // ---- radial groups -----------------------------------------------------
     const int n_rg = uniform(tab->n_rg);
@@ -755,7 +757,6 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
             if (MODE == MODE_ADJ) {
                 const R sc = dsum * (compact_adj<R, MODE>() ? M<R>::rcp(pv.w) : pv.w);
                 const R gx = sc * pv.x, gy = sc * pv.y, gz = sc * pv.z;
-                cgx += gx; cgy += gy; cgz += gz;
                 S.DA[s] -= (double)gx; S.DA[cap + s] -= (double)gy; S.DA[2 * cap + s] -= (double)gz;
             }
         }
@@ -823,31 +824,31 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
         wave_sync();
         if constexpr (KIND == PT_ACSF_TANH_P4) {
             if (ck == CUTK_TANH && ak != ANGK_GEN)
-                angular_group<R, MODE, CUTK_TANH, ANGK_P4>(S, X, W, acc, cgx, cgy, cgz, cap);
+                angular_group<R, MODE, CUTK_TANH, ANGK_P4>(S, X, W, acc, cap);
             else
                 bad = true;
         } else if constexpr (KIND == PT_ACSF_TANH_Z1N2) {
             if (ck == CUTK_TANH && ak == ANGK_Z1N2)
-                angular_group<R, MODE, CUTK_TANH, ANGK_Z1N2>(S, X, W, acc, cgx, cgy, cgz, cap);
+                angular_group<R, MODE, CUTK_TANH, ANGK_Z1N2>(S, X, W, acc, cap);
             else
                 bad = true;
         } else if (MODE == MODE_TAN || (sizeof(R) == 8 && ck != CUTK_TANH)) {
-            angular_group<R, MODE, CUTK_GEN, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
+            angular_group<R, MODE, CUTK_GEN, ANGK_GEN>(S, X, W, acc, cap);
         } else if (sizeof(R) == 8) {
-            if (ak == ANGK_GEN) angular_group<R, MODE, CUTK_TANH, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
-            else angular_group<R, MODE, CUTK_TANH, ANGK_P4>(S, X, W, acc, cgx, cgy, cgz, cap);
+            if (ak == ANGK_GEN) angular_group<R, MODE, CUTK_TANH, ANGK_GEN>(S, X, W, acc, cap);
+            else angular_group<R, MODE, CUTK_TANH, ANGK_P4>(S, X, W, acc, cap);
         } else if (ck == CUTK_TANH) {
-            if (ak == ANGK_Z1N2) angular_group<R, MODE, CUTK_TANH, ANGK_Z1N2>(S, X, W, acc, cgx, cgy, cgz, cap);
-            else if (ak == ANGK_Z1) angular_group<R, MODE, CUTK_TANH, ANGK_Z1>(S, X, W, acc, cgx, cgy, cgz, cap);
-            else if (ak == ANGK_P4) angular_group<R, MODE, CUTK_TANH, ANGK_P4>(S, X, W, acc, cgx, cgy, cgz, cap);
-            else angular_group<R, MODE, CUTK_TANH, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
+            if (ak == ANGK_Z1N2) angular_group<R, MODE, CUTK_TANH, ANGK_Z1N2>(S, X, W, acc, cap);
+            else if (ak == ANGK_Z1) angular_group<R, MODE, CUTK_TANH, ANGK_Z1>(S, X, W, acc, cap);
+            else if (ak == ANGK_P4) angular_group<R, MODE, CUTK_TANH, ANGK_P4>(S, X, W, acc, cap);
+            else angular_group<R, MODE, CUTK_TANH, ANGK_GEN>(S, X, W, acc, cap);
         } else if (ck == CUTK_COS) {
-            if (ak == ANGK_Z1N2) angular_group<R, MODE, CUTK_COS, ANGK_Z1N2>(S, X, W, acc, cgx, cgy, cgz, cap);
-            else if (ak == ANGK_Z1) angular_group<R, MODE, CUTK_COS, ANGK_Z1>(S, X, W, acc, cgx, cgy, cgz, cap);
-            else if (ak == ANGK_P4) angular_group<R, MODE, CUTK_COS, ANGK_P4>(S, X, W, acc, cgx, cgy, cgz, cap);
-            else angular_group<R, MODE, CUTK_COS, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
+            if (ak == ANGK_Z1N2) angular_group<R, MODE, CUTK_COS, ANGK_Z1N2>(S, X, W, acc, cap);
+            else if (ak == ANGK_Z1) angular_group<R, MODE, CUTK_COS, ANGK_Z1>(S, X, W, acc, cap);
+            else if (ak == ANGK_P4) angular_group<R, MODE, CUTK_COS, ANGK_P4>(S, X, W, acc, cap);
+            else angular_group<R, MODE, CUTK_COS, ANGK_GEN>(S, X, W, acc, cap);
         } else {
-            angular_group<R, MODE, CUTK_GEN, ANGK_GEN>(S, X, W, acc, cgx, cgy, cgz, cap);
+            angular_group<R, MODE, CUTK_GEN, ANGK_GEN>(S, X, W, acc, cap);
         }
 
         if (MODE != MODE_ADJ) {
@@ -904,11 +905,19 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
             }
         }
         if (a.f_self) {
-            const double sx = wave_sum((double)cgx), sy = wave_sum((double)cgy), sz = wave_sum((double)cgz);
+            // G_i depends on r_i only through R_ij = r_i - r_j, so dE_i/dr_i = -sum_j dE_i/dr_j
+            // and F_self = -dE_i/dr_i = sum_j dE_i/dr_j, the per-edge accumulators
+            double fx = 0.0, fy = 0.0, fz = 0.0;
+            for (int s = lane; s < m; s += kWave) {
+                fx += S.DA[s];
+                fy += S.DA[cap + s];
+                fz += S.DA[2 * cap + s];
+            }
+            const double sx = wave_sum(fx), sy = wave_sum(fy), sz = wave_sum(fz);
             if (lane == 0) {
-                a.f_self[3 * c + 0] = bad ? (double)nan : -sx;
-                a.f_self[3 * c + 1] = bad ? (double)nan : -sy;
-                a.f_self[3 * c + 2] = bad ? (double)nan : -sz;
+                a.f_self[3 * c + 0] = bad ? (double)nan : sx;
+                a.f_self[3 * c + 1] = bad ? (double)nan : sy;
+                a.f_self[3 * c + 2] = bad ? (double)nan : sz;
             }
         }
     }

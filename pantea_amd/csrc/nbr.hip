@@ -12,6 +12,15 @@
 // Rows are emitted in ascending internal index: the stencil cells are
 // visited in ascending cell id, type by type, and internal order is sorted
 // by (type, cell) with ties kept in input order (stable sort).
+//
+// The cutoff test r <= rc is done as s <= T on s = dx^2 + dy^2 + dz^2
+// (same operation order, no contraction), T being the largest double whose
+// correctly rounded square root is <= rc (found on the host): sqrt is
+// monotone, so this is the reference's test bit for bit without a sqrt per
+// candidate.  One candidate pass stages each row (up to kStageW entries) and
+// counts it; after the row-offset scan one kernel copies the staged rows into
+// the CSR arrays and pairs every edge with its reverse.  A list whose longest
+// row exceeds kStageW takes a second candidate pass instead.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -29,6 +38,9 @@ constexpr int kTile = kSortThreads * kSortItems;
 constexpr int kMaxTypes = 8;
 constexpr int kMaxRanges = 27 * kMaxTypes;
 constexpr int kScanBlock = 1024;  // items per block of the row-offset scan
+constexpr int kStageW = 256;      // staged entries per row
+constexpr int kBucket = 16;       // entries per bucket of a staged row
+constexpr int kNBucket = kStageW / kBucket;
 
 struct Grid {
     int nc[3];
@@ -81,9 +93,11 @@ __global__ void k_bbox(int n, const double* pos, int ps, double* out) {
         for (int d = 0; d < 3; ++d) { out[d] = smin[d][0]; out[3 + d] = smax[d][0]; }
 }
 
-__global__ void k_bin(int n, const double* pos, int ps, const int* types, Grid g, int* keys, int* vals) {
+__global__ void k_bin(int n, const double* pos, int ps, const int* types, Grid g, int* keys, int* vals,
+                      unsigned long long* bmax) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (i == 0) *bmax = 0ull;   // max |coordinate|, raised by k_gather_pos4
     int c0 = cell_coord(pos[(size_t)ps * i + 0], g.lo[0], g.inv_w[0], g.nc[0], g.periodic);
     int c1 = cell_coord(pos[(size_t)ps * i + 1], g.lo[1], g.inv_w[1], g.nc[1], g.periodic);
     int c2 = cell_coord(pos[(size_t)ps * i + 2], g.lo[2], g.inv_w[2], g.nc[2], g.periodic);
@@ -186,115 +200,290 @@ __global__ void k_cell_start(int nkeys, int n, const int* skeys, int* cell_start
     cell_start[k] = lo;
 }
 
-__global__ void k_gather_pos4(int n, const int* perm, const double* pos, int ps, const int* types,
-                              double* pos4) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int v = perm[i];
-    double4 p;
-    p.x = pos[(size_t)ps * v]; p.y = pos[(size_t)ps * v + 1]; p.z = pos[(size_t)ps * v + 2];
-    p.w = (double)types[v];
-    reinterpret_cast<double4*>(pos4)[i] = p;
+// Internal-order positions: FP64 (x, y, z, type) and an FP32 copy for the
+// candidate screen; bmax gets the bit pattern of max |coordinate| (the bit
+// patterns of non-negative doubles order like the values).
+__global__ __launch_bounds__(256) void k_gather_pos4(int n, const int* perm, const double* pos, int ps,
+                                                     const int* types, double* pos4, float4* pos4f,
+                                                     unsigned long long* bmax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (i < n) {
+        const int v = perm[i];
+        double4 p;
+        p.x = pos[(size_t)ps * v]; p.y = pos[(size_t)ps * v + 1]; p.z = pos[(size_t)ps * v + 2];
+        p.w = (double)types[v];
+        reinterpret_cast<double4*>(pos4)[i] = p;
+        pos4f[i] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
+        // a NaN coordinate makes the bound NaN: every candidate then takes the exact test
+        m = fmax(fmax(fabs(p.x), fabs(p.y)), fabs(p.z));
+        if (p.x != p.x || p.y != p.y || p.z != p.z) m = __builtin_nan("");
+    }
+    unsigned long long b = __builtin_bit_cast(unsigned long long, m) & ~(1ull << 63);
+    for (int off = 1; off < kWave; off <<= 1) {
+        const unsigned long long o = __shfl_xor(b, off, kWave);
+        b = o > b ? o : b;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0 && b != 0ull) atomicMax(bmax, b);
 }
 
-// ---- one wavefront per centre: count (FILL=false) or fill (FILL=true) -----
+// ---- one wavefront per centre: count, fill (CSR) or stage ------------------
 
-template <bool FILL>
-__global__ __launch_bounds__(256) void k_nbr_pass(int n, const double* __restrict__ pos4,
-                                                  const int* __restrict__ skeys,
+enum { PASS_FILL = 1, PASS_STAGE = 2 };
+
+// squared single-shift minimum-image distance, operation order of
+// atoms/distance.py:63-78 (no contraction)
+__device__ __forceinline__ double mic_dist2(double dx, double dy, double dz, const BoxArg& box) {
+#pragma clang fp contract(off)
+    if (box.periodic) {
+        dx = mic1(dx, box.L[0], box.half[0]);
+        dy = mic1(dy, box.L[1], box.half[1]);
+        dz = mic1(dz, box.L[2], box.half[2]);
+    }
+    double s = dx * dx;
+    s = s + dy * dy;
+    s = s + dz * dz;
+    return s;
+}
+
+// The neighbouring float of f towards +inf (up) or -inf; f is not NaN.
+__device__ __forceinline__ float f32_step(float f, bool up) {
+    if (f == 0.0f) return up ? 0x1p-149f : -0x1p-149f;
+    int b = __builtin_bit_cast(int, f);
+    b += ((f > 0.0f) == up) ? 1 : -1;
+    return __builtin_bit_cast(float, b);
+}
+
+// Conservative FP32 bounds of a double (below / above).
+__device__ __forceinline__ float f32_below(double x) {
+    const float f = (float)x;
+    return (double)f > x ? f32_step(f, false) : f;
+}
+
+__device__ __forceinline__ float f32_above(double x) {
+    const float f = (float)x;
+    return (double)f < x ? f32_step(f, true) : f;
+}
+
+// FP32 screen of a candidate: s32 is x_i - x_j, imaged and squared in FP32
+// from FP32 copies of the positions.  With u = 2^-24, B = max |coordinate|
+// and L the largest box length, each |d| component is within
+// delta = 16 u (B + L) of its FP64 value (rounding of both positions, the
+// difference, the box length and the shift, and an image branch taken
+// differently within that distance of L/2, where |d| and |d - L| agree), so
+// |s32 - s| <= delta (2 sqrt(3 s) + 3 delta) + 4 u s.  With M twice that
+// bound at s = T, s32 in (M, T - M) is a hit and s32 >= T + M a miss for
+// certain; anything else (and NaN) is settled by the FP64 test.
+struct Screen {
+    float hit_lo, hit_hi, miss;
+};
+
+__device__ __forceinline__ Screen make_screen(double thr, double bmax, const BoxArg& box) {
+    const double u = 0x1p-24;
+    const double L = box.periodic ? fmax(fmax(box.L[0], box.L[1]), box.L[2]) : 0.0;
+    const double dl = 16.0 * u * (bmax + L);
+    const double M = 2.0 * (dl * (2.0 * sqrt(3.0 * thr) + 3.0 * dl) + 4.0 * u * thr);
+    Screen sc;
+    sc.hit_lo = f32_above(M);
+    sc.hit_hi = f32_below(thr - M);
+    sc.miss = f32_above(thr + M);
+    if (!(M == M)) { sc.hit_lo = INFINITY; sc.hit_hi = -INFINITY; sc.miss = INFINITY; }
+    return sc;
+}
+
+__device__ __forceinline__ float mic_f(float d, float L, float half) {
+    d = (d > half) ? d - L : d;
+    d = (d < -half) ? d + L : d;
+    return d;
+}
+
+constexpr int kCand = 1536;   // candidates of one cell staged in LDS
+
+// One workgroup per cell: the candidate ranges of the cell's stencil (the
+// same for all its centres) are built once, the candidates (FP32 position and
+// index) staged in LDS when they fit, and the four wavefronts take the
+// cell's centres in turn, 64 candidates per step.  A cell with more than
+// kCand candidates reads them from global memory instead (index by binary
+// search over the ranges).  Rows come out in candidate order: type-major,
+// stencil cells ascending, i.e. ascending internal index.
+template <int PASS>
+__global__ __launch_bounds__(256) void k_nbr_cell(const double* __restrict__ pos4,
+                                                  const float4* __restrict__ pos4f,
+                                                  const unsigned long long* __restrict__ bmax,
                                                   const int* __restrict__ cell_start, Grid g,
-                                                  BoxArg box, int n_types, double rc,
+                                                  BoxArg box, int n_types, double thr,
                                                   int* __restrict__ counts,
                                                   const long long* __restrict__ row_off,
-                                                  int* __restrict__ nbr) {
-    __shared__ int s_cells[4][32];
-    __shared__ int s_rstart[4][kMaxRanges];
-    __shared__ int s_rpre[4][kMaxRanges + 1];
+                                                  int* __restrict__ out_idx,
+                                                  int* __restrict__ heads) {
+    // out_idx: the CSR neighbour array (FILL) or the n x kStageW stage (STAGE);
+    // heads (STAGE): the first entry of each bucket of kBucket staged entries
+    constexpr bool FILL = PASS == PASS_FILL;
+    __shared__ int4 s_cand[kCand];            // (x, y, z) as FP32 bits, internal index
+    __shared__ int s_cells[32];
+    __shared__ int s_runf[32], s_runl[32];    // first / last cell of each run
+    __shared__ int s_rstart[kMaxRanges];
+    __shared__ int s_rpre[kMaxRanges + 1];
+    __shared__ int s_cpre[kMaxTypes + 1];     // centres of this cell, by type
+    __shared__ int s_cbeg[kMaxTypes];
+    __shared__ int s_nr;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x >> 6;
-    const int i = uniform(blockIdx.x * 4 + w);
-    if (i >= n) return;
+    const int cell = blockIdx.x;
 
-    const double4 pi = reinterpret_cast<const double4*>(pos4)[i];
-    const int cell = uniform(skeys[i] % g.ncell);
-    const int c0 = cell % g.nc[0];
-    const int c1 = (cell / g.nc[0]) % g.nc[1];
-    const int c2 = cell / (g.nc[0] * g.nc[1]);
-
-    // stencil cells of this centre, sorted ascending
-    const int ns = g.nst[0] * g.nst[1] * g.nst[2];
-    int myc = -1;
-    if (lane < ns) {
-        int a = lane % g.nst[0], b = (lane / g.nst[0]) % g.nst[1], c = lane / (g.nst[0] * g.nst[1]);
-        int x = c0 + g.off[0][a], y = c1 + g.off[1][b], z = c2 + g.off[2][c];
-        bool ok = true;
-        if (g.periodic) {
-            x = (x + g.nc[0]) % g.nc[0]; y = (y + g.nc[1]) % g.nc[1]; z = (z + g.nc[2]) % g.nc[2];
-        } else {
-            ok = x >= 0 && x < g.nc[0] && y >= 0 && y < g.nc[1] && z >= 0 && z < g.nc[2];
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        s_cpre[0] = 0;
+        for (int t = 0; t < n_types; ++t) {
+            const int k = t * g.ncell + cell;
+            s_cbeg[t] = cell_start[k];
+            acc += cell_start[k + 1] - cell_start[k];
+            s_cpre[t + 1] = acc;
         }
-        myc = ok ? (z * g.nc[1] + y) * g.nc[0] + x : -1;
     }
-    int rank = 0;
-    for (int m = 0; m < ns; ++m) {
-        int o = __shfl(myc, m, kWave);
-        rank += (o >= 0 && o < myc) ? 1 : 0;
-    }
-    const int nvalid = __popcll(__ballot(myc >= 0));
-    if (myc >= 0) s_cells[w][rank] = myc;
-    wave_sync();
+    __syncthreads();
+    const int ncent = s_cpre[n_types];
+    if (ncent == 0) return;
 
-    // candidate ranges: type-major, cells ascending
-    const int nr = n_types * nvalid;
-    for (int r = lane; r < nr; r += kWave) {
-        int t = r / nvalid, q = r - t * nvalid;
-        int k = t * g.ncell + s_cells[w][q];
-        int st = cell_start[k];
-        s_rstart[w][r] = st;
-        s_rpre[w][r + 1] = cell_start[k + 1] - st;
-    }
-    if (lane == 0) s_rpre[w][0] = 0;
-    wave_sync();
-    // inclusive prefix over s_rpre[1..nr]
-    int carry = 0;
-    for (int base = 0; base < nr; base += kWave) {
-        int r = base + lane;
-        int v = r < nr ? s_rpre[w][r + 1] : 0;
-        int incl = v;
+    if (w == 0) {
+        const int c0 = cell % g.nc[0];
+        const int c1 = (cell / g.nc[0]) % g.nc[1];
+        const int c2 = cell / (g.nc[0] * g.nc[1]);
+        // stencil cells, sorted ascending
+        const int ns = g.nst[0] * g.nst[1] * g.nst[2];
+        int myc = -1;
+        if (lane < ns) {
+            int a = lane % g.nst[0], b = (lane / g.nst[0]) % g.nst[1], c = lane / (g.nst[0] * g.nst[1]);
+            int x = c0 + g.off[0][a], y = c1 + g.off[1][b], z = c2 + g.off[2][c];
+            bool ok = true;
+            if (g.periodic) {
+                x = (x + g.nc[0]) % g.nc[0]; y = (y + g.nc[1]) % g.nc[1]; z = (z + g.nc[2]) % g.nc[2];
+            } else {
+                ok = x >= 0 && x < g.nc[0] && y >= 0 && y < g.nc[1] && z >= 0 && z < g.nc[2];
+            }
+            myc = ok ? (z * g.nc[1] + y) * g.nc[0] + x : -1;
+        }
+        int rank = 0;
+        for (int m = 0; m < ns; ++m) {
+            int o = __shfl(myc, m, kWave);
+            rank += (o >= 0 && o < myc) ? 1 : 0;
+        }
+        const int nvalid = __popcll(__ballot(myc >= 0));
+        if (myc >= 0) s_cells[rank] = myc;
+        wave_sync();
+        // runs of consecutive cell ids (x-adjacent cells) are contiguous per type
+        const bool in = lane < nvalid;
+        const int cv = in ? s_cells[lane] : 0;
+        const bool first = in && (lane == 0 || s_cells[lane - 1] != cv - 1);
+        const bool last = in && (lane + 1 == nvalid || s_cells[lane + 1] != cv + 1);
+        const unsigned long long fm = __ballot(first);
+        const int nrun = __popcll(fm);
+        const int run = __popcll(fm & ((2ull << lane) - 1ull)) - 1;   // lane < 32
+        if (first) s_runf[run] = cv;
+        if (last) s_runl[run] = cv;
+        wave_sync();
+        // ranges (type-major, runs ascending) and their inclusive prefix
+        const int nr = n_types * nrun;
+        int carry = 0;
+        for (int base = 0; base < nr; base += kWave) {
+            const int r = base + lane;
+            int st = 0, len = 0;
+            if (r < nr) {
+                const int t = r / nrun, q = r - (r / nrun) * nrun;
+                const int k0 = t * g.ncell + s_runf[q], k1 = t * g.ncell + s_runl[q] + 1;
+                st = cell_start[k0];
+                len = cell_start[k1] - st;
+            }
+            int incl = len;
 #pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-            int t = __shfl_up(incl, off, kWave);
-            if (lane >= off) incl += t;
+            for (int off = 1; off < kWave; off <<= 1) {
+                const int o = __shfl_up(incl, off, kWave);
+                if (lane >= off) incl += o;
+            }
+            if (r < nr) { s_rstart[r] = st; s_rpre[r + 1] = carry + incl; }
+            carry += __shfl(incl, kWave - 1, kWave);
         }
-        if (r < nr) s_rpre[w][r + 1] = carry + incl;
-        carry += __shfl(incl, kWave - 1, kWave);
+        if (lane == 0) { s_rpre[0] = 0; s_nr = nr; }
     }
-    wave_sync();
-    const int total = carry;
-
-    long long out = FILL ? row_off[i] : 0;
-    int count = 0;
-    for (int base = 0; base < total; base += kWave) {
-        int t = base + lane;
-        bool hit = false;
-        int j = 0;
-        if (t < total) {
+    __syncthreads();
+    const int nr = s_nr;
+    const int total = s_rpre[nr];
+    const bool staged = total <= kCand;
+    if (staged) {
+        for (int t = threadIdx.x; t < total; t += 256) {
             int lo = 0, hi = nr;  // largest r with s_rpre[r] <= t
             while (hi - lo > 1) {
-                int mid = (lo + hi) >> 1;
-                if (s_rpre[w][mid] <= t) lo = mid; else hi = mid;
+                const int mid = (lo + hi) >> 1;
+                if (s_rpre[mid] <= t) lo = mid; else hi = mid;
             }
-            j = s_rstart[w][lo] + (t - s_rpre[w][lo]);
-            const double4 pj = reinterpret_cast<const double4*>(pos4)[j];
-            double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            double r = exact_dist(dx, dy, dz, box);
-            hit = (r <= rc) && (r > 0.0);
+            const int j = s_rstart[lo] + (t - s_rpre[lo]);
+            const float4 p = pos4f[j];
+            s_cand[t] = make_int4(__builtin_bit_cast(int, p.x), __builtin_bit_cast(int, p.y),
+                                  __builtin_bit_cast(int, p.z), j);
         }
-        unsigned long long bal = __ballot(hit);
-        if (FILL && hit) nbr[out + count + lanes_below(bal)] = j;
-        count += __popcll(bal);
     }
-    if (!FILL && lane == 0) counts[i] = count;
+    __syncthreads();
+
+    const Screen sc = make_screen(thr, __builtin_bit_cast(double, *bmax), box);
+    const float3 bf = make_float3((float)box.L[0], (float)box.L[1], (float)box.L[2]);
+    const float3 bh = make_float3((float)box.half[0], (float)box.half[1], (float)box.half[2]);
+    for (int k = w; k < ncent; k += 4) {
+        int t0 = 0;
+        while (t0 + 1 < n_types && s_cpre[t0 + 1] <= k) ++t0;
+        const int i = uniform(s_cbeg[t0] + (k - s_cpre[t0]));
+        const double4 pi = reinterpret_cast<const double4*>(pos4)[i];
+        const float4 pif = pos4f[i];
+        const long long out = FILL ? row_off[i] : 0;
+        int count = 0;
+        for (int base = 0; base < total; base += kWave) {
+            const int t = base + lane;
+            bool hit = false;
+            int j = i;
+            float4 pj = pif;
+            if (t < total) {
+                if (staged) {
+                    const int4 c = s_cand[t];
+                    pj = make_float4(__builtin_bit_cast(float, c.x), __builtin_bit_cast(float, c.y),
+                                     __builtin_bit_cast(float, c.z), 0.0f);
+                    j = c.w;
+                } else {
+                    int lo = 0, hi = nr;
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (s_rpre[mid] <= t) lo = mid; else hi = mid;
+                    }
+                    j = s_rstart[lo] + (t - s_rpre[lo]);
+                    pj = pos4f[j];
+                }
+            }
+            if (j != i) {   // r_ii = 0 is never a neighbour
+                float dx = pif.x - pj.x, dy = pif.y - pj.y, dz = pif.z - pj.z;
+                if (box.periodic) {
+                    dx = mic_f(dx, bf.x, bh.x);
+                    dy = mic_f(dy, bf.y, bh.y);
+                    dz = mic_f(dz, bf.z, bh.z);
+                }
+                const float s32 = dx * dx + dy * dy + dz * dz;
+                hit = (s32 > sc.hit_lo) & (s32 < sc.hit_hi);
+                if (!hit && !(s32 >= sc.miss)) {
+                    const double4 pd = reinterpret_cast<const double4*>(pos4)[j];
+                    const double s = mic_dist2(pi.x - pd.x, pi.y - pd.y, pi.z - pd.z, box);
+                    hit = (s <= thr) && (s > 0.0);   // r <= rc and r > 0
+                }
+            }
+            const unsigned long long bal = __ballot(hit);
+            if (hit) {
+                const int slot = count + lanes_below(bal);
+                if (FILL) out_idx[out + slot] = j;
+                if (PASS == PASS_STAGE && slot < kStageW) {
+                    out_idx[(size_t)i * kStageW + slot] = j;
+                    if (slot % kBucket == 0) heads[(size_t)i * kNBucket + slot / kBucket] = j;
+                }
+            }
+            count += __popcll(bal);
+        }
+        if (!FILL && lane == 0) counts[i] = count;
+    }
 }
 
 // ---- row offsets: exclusive scan of counts (int64) + max row ---------------
@@ -405,6 +594,78 @@ __global__ __launch_bounds__(256) void k_rev(int n, const long long* __restrict_
     }
 }
 
+// Staged rows -> CSR, and the reverse index of every edge.  Wave i copies
+// its row; for each j > i (a suffix of the ascending row) it finds i in the
+// staged row of j and writes both rev entries, so each unordered pair is
+// searched once.  The search is two 64-byte reads: the bucket heads of row j
+// (first entry of every kBucket entries) select the bucket, then the bucket
+// itself.  The relation is symmetric bit for bit (x_i - x_j and the
+// single-shift image are exactly antisymmetric), so every entry is written.
+__device__ __forceinline__ int count_lt(const int4& v, int base, int lim, int key) {
+    return (base + 0 < lim && v.x < key) + (base + 1 < lim && v.y < key) + (base + 2 < lim && v.z < key) +
+           (base + 3 < lim && v.w < key);
+}
+
+__device__ __forceinline__ int count_le(const int4& v, int base, int lim, int key) {
+    return (base + 0 < lim && v.x <= key) + (base + 1 < lim && v.y <= key) + (base + 2 < lim && v.z <= key) +
+           (base + 3 < lim && v.w <= key);
+}
+
+__device__ __forceinline__ bool has_eq(const int4& v, int base, int lim, int key) {
+    return (base + 0 < lim && v.x == key) || (base + 1 < lim && v.y == key) || (base + 2 < lim && v.z == key) ||
+           (base + 3 < lim && v.w == key);
+}
+
+__global__ __launch_bounds__(256) void k_compact_rev(int n, const long long* __restrict__ row_off,
+                                                     const int* __restrict__ counts,
+                                                     const int* __restrict__ stage,
+                                                     const int* __restrict__ heads, int* __restrict__ nbr,
+                                                     int* __restrict__ rev) {
+    static_assert(kBucket == 16 && kStageW % kBucket == 0, "four int4 per bucket");
+    const int lane = threadIdx.x & (kWave - 1);
+    const int i = uniform(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (i >= n) return;
+    const long long oi = row_off[i];
+    const int ci = counts[i];
+    const int* si = stage + (size_t)i * kStageW;
+    int below = 0;   // entries j < i
+    for (int base = 0; base < ci; base += kWave) {
+        const int t = base + lane;
+        const bool in = t < ci;
+        const int j = in ? si[t] : 0;
+        if (in) nbr[oi + t] = j;
+        below += __popcll(__ballot(in && j < i));
+    }
+    if (rev == nullptr) return;
+    for (int t = below + lane; t < ci; t += kWave) {
+        const int j = si[t];   // > i
+        const int cj = counts[j];
+        const int nb = (cj + kBucket - 1) / kBucket;
+        const int4* hp = reinterpret_cast<const int4*>(heads + (size_t)j * kNBucket);
+        const int4 h0 = hp[0], h1 = hp[1], h2 = hp[2], h3 = hp[3];
+        const int b = count_le(h0, 0, nb, i) + count_le(h1, 4, nb, i) + count_le(h2, 8, nb, i) +
+                      count_le(h3, 12, nb, i) - 1;   // bucket whose head is the last <= i
+        bool found = false;
+        int p = 0;
+        if (b >= 0) {
+            const int o = b * kBucket;
+            const int4* bp = reinterpret_cast<const int4*>(stage + (size_t)j * kStageW + o);
+            const int4 v0 = bp[0], v1 = bp[1], v2 = bp[2], v3 = bp[3];
+            p = o + count_lt(v0, o, cj, i) + count_lt(v1, o + 4, cj, i) + count_lt(v2, o + 8, cj, i) +
+                count_lt(v3, o + 12, cj, i);
+            found = has_eq(v0, o, cj, i) || has_eq(v1, o + 4, cj, i) || has_eq(v2, o + 8, cj, i) ||
+                    has_eq(v3, o + 12, cj, i);
+        }
+        if (found) {
+            const long long oj = row_off[j];
+            rev[oi + t] = (int)(oj + p);
+            rev[oj + p] = (int)(oi + t);
+        } else {
+            rev[oi + t] = -1;   // only if the list were asymmetric
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_mark_rows(int n_sel, const int* __restrict__ sel,
                                                    const long long* __restrict__ row_off,
                                                    const int* __restrict__ nbr, unsigned char* __restrict__ flags) {
@@ -417,7 +678,7 @@ __global__ __launch_bounds__(256) void k_mark_rows(int n_sel, const int* __restr
 }
 
 struct WsLayout {
-    size_t keys_a, keys_b, vals_a, vals_b, ghist, cell_start, counts, psum, pmax, misc, total;
+    size_t keys_a, keys_b, vals_a, vals_b, ghist, cell_start, counts, psum, pmax, misc, stage, bucket, pos4f, total;
 };
 
 inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
@@ -437,6 +698,9 @@ WsLayout ws_layout(int n, int n_types) {
     L.psum = o; o = align_up(o + sizeof(long long) * (nsb + 1));
     L.pmax = o; o = align_up(o + sizeof(int) * (nsb + 1));
     L.misc = o; o = align_up(o + 64 * sizeof(double));
+    L.stage = o; o = align_up(o + sizeof(int) * (size_t)n * kStageW);
+    L.bucket = o; o = align_up(o + sizeof(int) * (size_t)n * kNBucket);
+    L.pos4f = o; o = align_up(o + sizeof(float4) * (size_t)n);
     L.total = o;
     return L;
 }
@@ -449,6 +713,18 @@ int cells_for(double extent, double rc) {
     int nc = (int)std::floor(extent / rc);
     while (nc > 1 && extent / nc <= rc * (1.0 + 1e-9)) --nc;
     return nc < 1 ? 1 : nc;
+}
+
+// Largest double T with sqrt(T) <= rc (IEEE sqrt is correctly rounded and
+// monotone, so sqrt(s) <= rc exactly when s <= T).
+double dist2_threshold(double rc) {
+    double t = rc * rc;
+    while (t > 0.0 && std::sqrt(t) > rc) t = std::nextafter(t, 0.0);
+    for (;;) {
+        const double u = std::nextafter(t, INFINITY);
+        if (std::sqrt(u) <= rc) t = u; else break;
+    }
+    return t;
 }
 
 }  // namespace
@@ -470,7 +746,7 @@ extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int*
     PT_REQUIRE(n >= 1, "pt_nbr_build: n must be >= 1 (got %d)", n);
     PT_REQUIRE(n_types >= 1 && n_types <= kMaxTypes, "pt_nbr_build: n_types must be in [1, %d]",
                kMaxTypes);
-    PT_REQUIRE(rc > 0.0, "pt_nbr_build: rc must be > 0");
+    PT_REQUIRE(rc > 0.0 && std::isfinite(rc), "pt_nbr_build: rc must be finite and > 0");
     PT_REQUIRE(pos && types && perm && pos4 && row_off && n_pairs && max_row && ws,
                "pt_nbr_build: null pointer argument");
     PT_REQUIRE(pos_stride >= 3, "pt_nbr_build: pos_stride must be >= 3");
@@ -489,6 +765,11 @@ extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int*
     long long* psum = (long long*)(w + L.psum);
     int* pmax = (int*)(w + L.pmax);
     double* misc = (double*)(w + L.misc);
+    int* stage = (int*)(w + L.stage);
+    int* heads = (int*)(w + L.bucket);
+    float4* pos4f = (float4*)(w + L.pos4f);
+    unsigned long long* bmax = (unsigned long long*)(misc + 8);   // misc[0..5]: bbox, then total / max row
+    const double thr = dist2_threshold(rc);
 
     Grid g{};
     BoxArg ba{};
@@ -535,7 +816,7 @@ extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int*
     // bin + sort
     const int tb = 256;
     hipLaunchKernelGGL(k_bin, dim3((n + tb - 1) / tb), dim3(tb), 0, stream, n, pos, pos_stride, types,
-                       g, keys_a, vals_a);
+                       g, keys_a, vals_a, bmax);
     if (int rcode = launch_check("k_bin")) return rcode;
     int bits = 1;
     while ((1 << bits) < nkeys) ++bits;
@@ -556,14 +837,15 @@ extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int*
     hipLaunchKernelGGL(k_cell_start, dim3((nkeys + 1 + tb - 1) / tb), dim3(tb), 0, stream, nkeys, n,
                        kin, cell_start);
     hipLaunchKernelGGL(k_gather_pos4, dim3((n + tb - 1) / tb), dim3(tb), 0, stream, n, vin, pos,
-                       pos_stride, types, pos4);
+                       pos_stride, types, pos4, pos4f, bmax);
     if (int rcode = launch_check("cell start / gather")) return rcode;
 
-    // count, scan, fill
+    // count + stage, scan, then compact (or a second pass for rows > kStageW)
     const int nwb = (n + 3) / 4;
-    hipLaunchKernelGGL(k_nbr_pass<false>, dim3(nwb), dim3(256), 0, stream, n, pos4, kin, cell_start,
-                       g, ba, n_types, rc, counts, (const long long*)nullptr, (int*)nullptr);
-    if (int rcode = launch_check("k_nbr_pass<count>")) return rcode;
+    hipLaunchKernelGGL(k_nbr_cell<PASS_STAGE>, dim3(g.ncell), dim3(256), 0, stream, pos4,
+                       (const float4*)pos4f, (const unsigned long long*)bmax, cell_start,
+                       g, ba, n_types, thr, counts, (const long long*)nullptr, stage, heads);
+    if (int rcode = launch_check("k_nbr_cell<stage>")) return rcode;
     const int nsb = (n + kScanBlock - 1) / kScanBlock;
     long long* tot_max = (long long*)misc;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(256), 0, stream, n, counts, psum, pmax);
@@ -581,9 +863,15 @@ extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int*
         return PT_ERR_CAPACITY;
     }
     PT_REQUIRE(nbr != nullptr || host_tm[0] == 0, "pt_nbr_build: nbr is null");
-    hipLaunchKernelGGL(k_nbr_pass<true>, dim3(nwb), dim3(256), 0, stream, n, pos4, kin, cell_start,
-                       g, ba, n_types, rc, counts, row_off, nbr);
-    if (int rcode = launch_check("k_nbr_pass<fill>")) return rcode;
+    if (host_tm[1] <= kStageW) {
+        hipLaunchKernelGGL(k_compact_rev, dim3(nwb), dim3(256), 0, stream, n, (const long long*)row_off,
+                           (const int*)counts, (const int*)stage, (const int*)heads, nbr, rev);
+        return launch_check("k_compact_rev");
+    }
+    hipLaunchKernelGGL(k_nbr_cell<PASS_FILL>, dim3(g.ncell), dim3(256), 0, stream, pos4,
+                       (const float4*)pos4f, (const unsigned long long*)bmax, cell_start,
+                       g, ba, n_types, thr, counts, row_off, nbr, (int*)nullptr);
+    if (int rcode = launch_check("k_nbr_cell<fill>")) return rcode;
     if (rev) {
         hipLaunchKernelGGL(k_rev, dim3(nwb), dim3(256), 0, stream, n, row_off, nbr, rev);
         if (int rcode = launch_check("k_rev")) return rcode;

@@ -55,7 +55,33 @@ def _water(name):
 # (a) neighbour list
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("case", ["cfg0_rc12", "box7_rc6", "box10_rc3", "cfg1_rc12", "cluster_nopbc"])
+def _rc_boundary_pairs(rc):
+    """72 isolated pairs whose separations are rc, rc +- a few ulps (axis, face
+    and body diagonals), in a periodic box of 60."""
+    pts = []
+    for p in range(72):
+        base = np.array([10.0 * (p % 6), 10.0 * ((p // 6) % 6), 10.0 * (p // 36)])
+        k, kind = p % 9 - 4, (p // 9) % 3
+        d = np.array([rc, 0.0, 0.0]) if kind == 0 else (
+            np.array([rc, rc, 0.0]) / np.sqrt(2.0) if kind == 1 else np.full(3, rc / np.sqrt(3.0)))
+        q = base + d
+        for _ in range(abs(k)):
+            q[0] = np.nextafter(q[0], np.inf if k > 0 else -np.inf)
+        pts += [base, q]
+    pos = np.array(pts)
+    dd = pos[1::2] - pos[0::2]
+    s2 = dd[:, 0] * dd[:, 0]
+    s2 = s2 + dd[:, 1] * dd[:, 1]
+    s2 = s2 + dd[:, 2] * dd[:, 2]
+    r = np.sqrt(s2)
+    # the set must hold separations exactly at rc and one ulp either side
+    assert np.any(r == rc) and np.any((r > rc) & (r < rc * (1 + 1e-15))) and np.any((r < rc) & (r > rc * (1 - 1e-15)))
+    return {"positions": pos, "elements": ["H"] * len(pos), "lattice": np.diag([60.0] * 3)}
+
+
+@pytest.mark.parametrize("case", ["cfg0_rc12", "box7_rc6", "box10_rc3", "cfg1_rc12", "cluster_nopbc",
+                                  "rc_boundary_ulps", "long_rows", "cluster_far_from_origin",
+                                  "clusters_many_candidates", "dense_many_candidates"])
 def test_neighbor_list_bit_exact(dev, case):
     from oracle import pyoracle
     from pantea_amd import water
@@ -72,6 +98,32 @@ def test_neighbor_list_bit_exact(dev, case):
     elif case == "box10_rc3":
         d = {"positions": rng.random((300, 3)) * 10.0, "elements": ["H"] * 300, "lattice": np.diag([10.0] * 3)}
         rc = 3.0
+    elif case == "rc_boundary_ulps":
+        rc = 3.7
+        d = _rc_boundary_pairs(rc)
+    elif case == "long_rows":
+        # ~480 neighbours per atom: longer than the staged row width, so the
+        # list takes the second candidate pass
+        d = {"positions": rng.random((1200, 3)) * 12.0, "elements": ["O", "H"] * 600, "lattice": np.diag([12.0] * 3)}
+        rc = 5.5
+    elif case == "clusters_many_candidates":
+        # 27 cells of width 10 > rc, each with a tight cluster of 60 atoms:
+        # 1620 candidates per cell (more than one workgroup stages in LDS)
+        # and rows of 59
+        centres = (np.stack(np.meshgrid(*[np.arange(3)] * 3, indexing="ij"), -1).reshape(-1, 1, 3) + 0.5) * 10.0
+        pos = (centres + rng.uniform(-0.2, 0.2, size=(27, 60, 3))).reshape(-1, 3)
+        d = {"positions": pos, "elements": ["O", "H"] * 810, "lattice": np.diag([30.0] * 3)}
+        rc = 9.0
+    elif case == "dense_many_candidates":
+        # 4000 candidates per cell and rows of ~1600
+        d = {"positions": rng.random((4000, 3)) * 12.0, "elements": ["H"] * 4000, "lattice": np.diag([12.0] * 3)}
+        rc = 5.5
+    elif case == "cluster_far_from_origin":
+        # coordinates near 5000: the FP32 candidate screen is coarse (ulp ~5e-4)
+        # and many candidates fall to the FP64 test
+        d = {"positions": 5000.0 + rng.normal(size=(1500, 3)) * 6.0, "elements": ["O", "H", "H"] * 500,
+             "lattice": []}
+        rc = 3.5
     else:
         d = {"positions": rng.normal(size=(150, 3)) * 4.0, "elements": ["O", "H", "H"] * 50, "lattice": []}
         rc = 3.5
@@ -95,6 +147,13 @@ def test_neighbor_list_bit_exact(dev, case):
     # internal order groups types
     t_int = s.atom_types_host[nl.perm_host]
     assert np.all(np.diff(t_int) >= 0)
+    assert nl.max_row == int(np.max(counts))
+    if case in ("long_rows", "dense_many_candidates"):
+        assert nl.max_row > 256
+    if case == "clusters_many_candidates":
+        assert nl.max_row == 59
+    if case == "rc_boundary_ulps":
+        assert 0 < nl.n_pairs < 144
 
 
 # ---------------------------------------------------------------------------

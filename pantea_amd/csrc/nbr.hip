@@ -123,27 +123,42 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(int n, const int* k
 }
 
 // Exclusive scan of m ints in one block (m up to a few 1e5).
+// Exclusive scan of m ints in place by one block: each thread scans a
+// contiguous chunk of ceil(m / 1024), the chunk totals are scanned across the
+// block (wave shuffles, then the 16 wave totals).
 __global__ __launch_bounds__(1024) void k_scan_small(int m, int* data) {
-    __shared__ int buf[1024];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
+    __shared__ int wtot[1024 / kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x >> 6;
+    const int per = (m + 1023) / 1024;
+    const int b = threadIdx.x * per;
+    const int e = min(b + per, m);
+    int sum = 0;
+    for (int k = b; k < e; ++k) sum += data[k];
+    int incl = sum;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int o = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += o;
+    }
+    if (lane == kWave - 1) wtot[w] = incl;
     __syncthreads();
-    for (int base = 0; base < m; base += 1024) {
-        int idx = base + threadIdx.x;
-        int v = idx < m ? data[idx] : 0;
-        buf[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            int t = (int)threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
-            __syncthreads();
-            buf[threadIdx.x] += t;
-            __syncthreads();
+    if (w == 0) {
+        int v = lane < 1024 / kWave ? wtot[lane] : 0;
+        int vi = v;
+#pragma unroll
+        for (int off = 1; off < 1024 / kWave; off <<= 1) {
+            const int o = __shfl_up(vi, off, kWave);
+            if (lane >= off) vi += o;
         }
-        int incl = buf[threadIdx.x];
-        if (idx < m) data[idx] = carry + incl - v;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry += buf[1023];
-        __syncthreads();
+        if (lane < 1024 / kWave) wtot[lane] = vi - v;   // exclusive over waves
+    }
+    __syncthreads();
+    int run = wtot[w] + incl - sum;
+    for (int k = b; k < e; ++k) {
+        const int v = data[k];
+        data[k] = run;
+        run += v;
     }
 }
 
@@ -224,7 +239,14 @@ __global__ __launch_bounds__(256) void k_gather_pos4(int n, const int* perm, con
         const unsigned long long o = __shfl_xor(b, off, kWave);
         b = o > b ? o : b;
     }
-    if ((threadIdx.x & (kWave - 1)) == 0 && b != 0ull) atomicMax(bmax, b);
+    // one atomic per block, and only when it raises the bound
+    __shared__ unsigned long long wmax[256 / kWave];
+    if ((threadIdx.x & (kWave - 1)) == 0) wmax[threadIdx.x >> 6] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) b = wmax[q] > b ? wmax[q] : b;
+        if (b > __hip_atomic_load(bmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(bmax, b);
+    }
 }
 
 // ---- one wavefront per centre: count, fill (CSR) or stage ------------------
@@ -269,8 +291,8 @@ __device__ __forceinline__ float f32_above(double x) {
 // from FP32 copies of the positions.  With u = 2^-24, B = max |coordinate|
 // and L the largest box length, each |d| component is within
 // delta = 16 u (B + L) of its FP64 value (rounding of both positions, the
-// difference, the box length and the shift, and an image branch taken
-// differently within that distance of L/2, where |d| and |d - L| agree), so
+// difference, the box length and the shift, and an image chosen differently
+// within that distance of L/2, where |d| and |d - L| agree), so
 // |s32 - s| <= delta (2 sqrt(3 s) + 3 delta) + 4 u s.  With M twice that
 // bound at s = T, s32 in (M, T - M) is a hit and s32 >= T + M a miss for
 // certain; anything else (and NaN) is settled by the FP64 test.
@@ -291,13 +313,7 @@ __device__ __forceinline__ Screen make_screen(double thr, double bmax, const Box
     return sc;
 }
 
-__device__ __forceinline__ float mic_f(float d, float L, float half) {
-    d = (d > half) ? d - L : d;
-    d = (d < -half) ? d + L : d;
-    return d;
-}
-
-constexpr int kCand = 1536;   // candidates of one cell staged in LDS
+constexpr int kCand = 1536;   // candidates of one cell staged in LDS (a multiple of 64)
 
 // One workgroup per cell: the candidate ranges of the cell's stencil (the
 // same for all its centres) are built once, the candidates (FP32 position and
@@ -425,8 +441,11 @@ __global__ __launch_bounds__(256) void k_nbr_cell(const double* __restrict__ pos
     __syncthreads();
 
     const Screen sc = make_screen(thr, __builtin_bit_cast(double, *bmax), box);
+    // FP32 image: d - L rint(d / L), the single-shift image whenever |d| < 1.5 L
+    // (a candidate with a larger component takes the FP64 test)
     const float3 bf = make_float3((float)box.L[0], (float)box.L[1], (float)box.L[2]);
-    const float3 bh = make_float3((float)box.half[0], (float)box.half[1], (float)box.half[2]);
+    const float3 ib = make_float3(1.0f / bf.x, 1.0f / bf.y, 1.0f / bf.z);
+    const float3 lim = make_float3(1.45f * bf.x, 1.45f * bf.y, 1.45f * bf.z);
     for (int k = w; k < ncent; k += 4) {
         int t0 = 0;
         while (t0 + 1 < n_types && s_cpre[t0 + 1] <= k) ++t0;
@@ -437,39 +456,38 @@ __global__ __launch_bounds__(256) void k_nbr_cell(const double* __restrict__ pos
         int count = 0;
         for (int base = 0; base < total; base += kWave) {
             const int t = base + lane;
-            bool hit = false;
             int j = i;
             float4 pj = pif;
-            if (t < total) {
-                if (staged) {
-                    const int4 c = s_cand[t];
-                    pj = make_float4(__builtin_bit_cast(float, c.x), __builtin_bit_cast(float, c.y),
-                                     __builtin_bit_cast(float, c.z), 0.0f);
-                    j = c.w;
-                } else {
-                    int lo = 0, hi = nr;
-                    while (hi - lo > 1) {
-                        const int mid = (lo + hi) >> 1;
-                        if (s_rpre[mid] <= t) lo = mid; else hi = mid;
-                    }
-                    j = s_rstart[lo] + (t - s_rpre[lo]);
-                    pj = pos4f[j];
+            if (staged) {
+                // t < kCand: total <= kCand and kCand is a multiple of the wave size
+                const int4 c = s_cand[t];
+                pj = make_float4(__builtin_bit_cast(float, c.x), __builtin_bit_cast(float, c.y),
+                                 __builtin_bit_cast(float, c.z), 0.0f);
+                j = c.w;
+            } else if (t < total) {
+                int lo = 0, hi = nr;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_rpre[mid] <= t) lo = mid; else hi = mid;
                 }
+                j = s_rstart[lo] + (t - s_rpre[lo]);
+                pj = pos4f[j];
             }
-            if (j != i) {   // r_ii = 0 is never a neighbour
-                float dx = pif.x - pj.x, dy = pif.y - pj.y, dz = pif.z - pj.z;
-                if (box.periodic) {
-                    dx = mic_f(dx, bf.x, bh.x);
-                    dy = mic_f(dy, bf.y, bh.y);
-                    dz = mic_f(dz, bf.z, bh.z);
-                }
-                const float s32 = dx * dx + dy * dy + dz * dz;
-                hit = (s32 > sc.hit_lo) & (s32 < sc.hit_hi);
-                if (!hit && !(s32 >= sc.miss)) {
-                    const double4 pd = reinterpret_cast<const double4*>(pos4)[j];
-                    const double s = mic_dist2(pi.x - pd.x, pi.y - pd.y, pi.z - pd.z, box);
-                    hit = (s <= thr) && (s > 0.0);   // r <= rc and r > 0
-                }
+            const bool ok = (t < total) & (j != i);   // r_ii = 0 is never a neighbour
+            float dx = pif.x - pj.x, dy = pif.y - pj.y, dz = pif.z - pj.z;
+            bool img = true;
+            if (box.periodic) {
+                img = (fabsf(dx) < lim.x) & (fabsf(dy) < lim.y) & (fabsf(dz) < lim.z);
+                dx = fmaf(-bf.x, rintf(dx * ib.x), dx);
+                dy = fmaf(-bf.y, rintf(dy * ib.y), dy);
+                dz = fmaf(-bf.z, rintf(dz * ib.z), dz);
+            }
+            const float s32 = dx * dx + dy * dy + dz * dz;
+            bool hit = ok & img & (s32 > sc.hit_lo) & (s32 < sc.hit_hi);
+            if (ok & !hit & (!img | !(s32 >= sc.miss))) {
+                const double4 pd = reinterpret_cast<const double4*>(pos4)[j];
+                const double s = mic_dist2(pi.x - pd.x, pi.y - pd.y, pi.z - pd.z, box);
+                hit = (s <= thr) && (s > 0.0);   // r <= rc and r > 0
             }
             const unsigned long long bal = __ballot(hit);
             if (hit) {

@@ -52,7 +52,9 @@ enum { PT_F64 = 0, PT_F32 = 1 };
 #define PT_ACSF_KIND_SHIFT 8
 enum { PT_ACSF_GENERIC = 0,     /* any table */
        PT_ACSF_TANH_P4 = 1,     /* every angular group: tanh/tanhu cutoff, zeta in {1, 2, 4} */
-       PT_ACSF_TANH_Z1N2 = 2 }; /* every angular group: tanh/tanhu cutoff, zeta = 1, <= 2 terms */
+       PT_ACSF_TANH_Z1N2 = 2,   /* every angular group: tanh/tanhu cutoff, zeta = 1, <= 2 terms */
+       PT_ACSF_TANH_Z14 = 3 };  /* every angular group: tanh/tanhu cutoff, exactly the four terms
+                                   zeta {1, 4} x lambda {+1, -1} */
 
 /* Arithmetic of the MLP kernels (matrix cores). */
 enum {

@@ -25,7 +25,7 @@ PT_ERR_UNSUPPORTED = -4
 PT_F64 = 0
 PT_F32 = 1
 PT_ACSF_KIND_SHIFT = 8
-PT_ACSF_GENERIC, PT_ACSF_TANH_P4, PT_ACSF_TANH_Z1N2 = 0, 1, 2
+PT_ACSF_GENERIC, PT_ACSF_TANH_P4, PT_ACSF_TANH_Z1N2, PT_ACSF_TANH_Z14 = 0, 1, 2, 3
 
 PT_MLP_F64 = 0
 PT_MLP_BF16 = 1

@@ -231,10 +231,13 @@ __device__ __forceinline__ void angular_term(R c, R lam, R zeta, int izeta, R pr
 
 // Angular-term evaluation specialised per group: ANGK_Z1 (every zeta = 1),
 // ANGK_Z1N2 (every zeta = 1, at most two terms), ANGK_P4 (every zeta in
-// {1, 2, 4}) or the generic square-and-multiply / pow.  The specialised
-// forms leave the factor pre = 2^(1 - zeta) out of the pair loop: it scales
-// the group's sums (forward) or dE/dG (adjoint) once per centre.
-enum { ANGK_Z1 = 0, ANGK_P4 = 1, ANGK_GEN = 2, ANGK_Z1N2 = 3 };
+// {1, 2, 4}), ANGK_Z14 (exactly the four terms zeta in {1, 4} x lambda in
+// {+1, -1}, evaluated in that canonical order from the shared powers of
+// 1 +- cos; the group's columns are mapped to it) or the generic
+// square-and-multiply / pow.  The specialised forms leave the factor
+// pre = 2^(1 - zeta) out of the pair loop: it scales the group's sums
+// (forward) or dE/dG (adjoint) once per centre.
+enum { ANGK_Z1 = 0, ANGK_P4 = 1, ANGK_GEN = 2, ANGK_Z1N2 = 3, ANGK_Z14 = 4 };
 
 template <int ANG>
 __host__ __device__ constexpr int ang_terms() { return ANG == ANGK_Z1N2 ? 2 : PT_AG_ANG; }
@@ -370,6 +373,17 @@ __device__ __forceinline__ void ang_eval(const AngGroup<R>& X, R cs, R (&v)[PT_A
     // specialised forms v = (1 + lam c)^zeta and dv = zeta lam (1 + lam c)^(zeta - 1)
     // without pre, in the generic one both include pre
     constexpr int NA = ang_terms<ANG>();
+    if constexpr (ANG == ANGK_Z14) {
+        // (1 + c), (1 - c), (1 + c)^4, (1 - c)^4 and their derivatives in c
+        const R bp = R(1) + cs, bm = R(1) - cs;
+        const R bp2 = bp * bp, bm2 = bm * bm;
+        v[0] = bp; v[1] = bm; v[2] = bp2 * bp2; v[3] = bm2 * bm2;
+        dv[0] = DER ? R(1) : R(0);
+        dv[1] = DER ? R(-1) : R(0);
+        dv[2] = DER ? R(4) * bp * bp2 : R(0);
+        dv[3] = DER ? R(-4) * bm * bm2 : R(0);
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < NA; ++q) {
         if (ANG == ANGK_Z1 || ANG == ANGK_Z1N2) {
@@ -627,6 +641,24 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
     wave_sync();
 }
 
+// 1 when the group's terms are exactly zeta {1, 4} x lambda {+1, -1} (any
+// order); qmap[k] then gives the term of canonical slot k = 2 [zeta = 4] +
+// [lambda = -1].
+__device__ __forceinline__ int z14_order(const pt_angular_group* G, int (&qmap)[PT_AG_ANG]) {
+    if (uniform(G->n_ang) != 4) return 0;
+    int seen = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int iz = uniform(G->izeta[q]);
+        const double lam = G->lam[q];
+        if (!((iz == 1 || iz == 4) && (lam == 1.0 || lam == -1.0))) return 0;
+        const int k = (iz == 4 ? 2 : 0) + (lam < 0.0 ? 1 : 0);
+        seen |= 1 << k;
+        qmap[k] = q;
+    }
+    return seen == 15 ? 1 : 0;
+}
+
 template <typename R, int MODE, int KIND>
 __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
     if (sizeof(R) == 8) exp2_tab_load();
@@ -793,6 +825,14 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
         X.cpre = X.cut == PT_CUT_TANH ? tanh_pre : R(1);
         X.tanh_pre = tanh_pre;
         X.box = &a.box;
+        // qmap[k]: the table term evaluated in slot k (the identity, or the
+        // canonical order of a ZETA {1, 4} x LAMBDA {+1, -1} group)
+        int qmap[PT_AG_ANG];
+        const int z14 = (KIND == PT_ACSF_TANH_Z14) ? z14_order(G, qmap) : 0;
+        if (!z14) {
+#pragma unroll
+            for (int q = 0; q < PT_AG_ANG; ++q) qmap[q] = q;
+        }
         R W[PT_AG_ETA][PT_AG_ANG], acc[PT_AG_ETA][PT_AG_ANG];
 #pragma unroll
         for (int e = 0; e < PT_AG_ETA; ++e) {
@@ -800,7 +840,7 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
 #pragma unroll
             for (int q = 0; q < PT_AG_ANG; ++q) {
                 acc[e][q] = R(0);
-                const int col = G->col[e * PT_AG_ANG + q];
+                const int col = G->col[e * PT_AG_ANG + qmap[q]];
                 W[e][q] = (MODE == MODE_ADJ && e < X.n_eta && q < X.n_ang && col >= 0)
                               ? w[(size_t)c * a.ldw + col]
                               : R(0);
@@ -810,19 +850,26 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
 #pragma unroll
         for (int q = 0; q < PT_AG_ANG; ++q) {
             const bool on = q < X.n_ang;
-            X.lam[q] = on ? (R)G->lam[q] : R(0);
-            X.zeta[q] = on ? (R)G->zeta[q] : R(1);
-            X.pre[q] = on ? (R)G->pre[q] : R(0);
-            X.izeta[q] = on ? uniform(G->izeta[q]) : 1;
-            X.zl[q] = on ? (R)(G->zeta[q] * G->lam[q]) : R(0);
+            const int tq = qmap[q];
+            X.lam[q] = on ? (R)G->lam[tq] : R(0);
+            X.zeta[q] = on ? (R)G->zeta[tq] : R(1);
+            X.pre[q] = on ? (R)G->pre[tq] : R(0);
+            X.izeta[q] = on ? uniform(G->izeta[tq]) : 1;
+            X.zl[q] = on ? (R)(G->zeta[tq] * G->lam[tq]) : R(0);
             z1 = z1 && X.izeta[q] == 1;
             p4 = p4 && (X.izeta[q] == 1 || X.izeta[q] == 2 || X.izeta[q] == 4);
         }
         const int ck = (X.cut == PT_CUT_TANH || X.cut == PT_CUT_TANHU) ? CUTK_TANH
                        : (X.cut == PT_CUT_COS ? CUTK_COS : CUTK_GEN);
-        const int ak = z1 ? (X.n_ang <= 2 ? ANGK_Z1N2 : ANGK_Z1) : (p4 ? ANGK_P4 : ANGK_GEN);
+        const int ak = z14 ? ANGK_Z14
+                           : (z1 ? (X.n_ang <= 2 ? ANGK_Z1N2 : ANGK_Z1) : (p4 ? ANGK_P4 : ANGK_GEN));
         wave_sync();
-        if constexpr (KIND == PT_ACSF_TANH_P4) {
+        if constexpr (KIND == PT_ACSF_TANH_Z14) {
+            if (ck == CUTK_TANH && ak == ANGK_Z14)
+                angular_group<R, MODE, CUTK_TANH, ANGK_Z14>(S, X, W, acc, cap);
+            else
+                bad = true;
+        } else if constexpr (KIND == PT_ACSF_TANH_P4) {
             if (ck == CUTK_TANH && ak != ANGK_GEN)
                 angular_group<R, MODE, CUTK_TANH, ANGK_P4>(S, X, W, acc, cap);
             else
@@ -859,7 +906,7 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
                 for (int q = 0; q < PT_AG_ANG; ++q)
                     if (e < X.n_eta && q < X.n_ang) {
                         const R v = wave_sum(acc[e][q]);
-                        const int col = G->col[e * PT_AG_ANG + q];
+                        const int col = G->col[e * PT_AG_ANG + qmap[q]];
                         if (lane == 0 && col >= 0) out[(size_t)c * a.ld + col] = v;
                     }
         }
@@ -981,6 +1028,9 @@ int run_kind(int mode, int kind, KArgs& a, hipStream_t st) {
     case PT_ACSF_TANH_Z1N2:
         return mode == MODE_FWD ? launch<R, MODE_FWD, PT_ACSF_TANH_Z1N2>(a, st)
                                 : launch<R, MODE_ADJ, PT_ACSF_TANH_Z1N2>(a, st);
+    case PT_ACSF_TANH_Z14:
+        return mode == MODE_FWD ? launch<R, MODE_FWD, PT_ACSF_TANH_Z14>(a, st)
+                                : launch<R, MODE_ADJ, PT_ACSF_TANH_Z14>(a, st);
     }
     set_error("acsf: unknown kernel kind %d", kind);
     return PT_ERR_ARG;
@@ -1067,17 +1117,24 @@ extern "C" int pt_acsf_tangent(int precision, const double* pos4, const double* 
 extern "C" int pt_sf_table_kind(const pt_sf_table* t) {
     if (!t) return PT_ACSF_GENERIC;
     if (t->n_ag <= 0) return PT_ACSF_GENERIC;
-    bool p4 = true, z1n2 = true;
+    bool p4 = true, z1n2 = true, z14 = true;
     for (int g = 0; g < t->n_ag && g < PT_MAX_GROUPS; ++g) {
         const pt_angular_group& G = t->ag[g];
         const bool tanh = G.cut == PT_CUT_TANH || G.cut == PT_CUT_TANHU;
         bool allz1 = true, allp4 = true;
+        int seen = 0;
         for (int q = 0; q < G.n_ang; ++q) {
             allz1 = allz1 && G.izeta[q] == 1;
             allp4 = allp4 && (G.izeta[q] == 1 || G.izeta[q] == 2 || G.izeta[q] == 4);
+            // the kernel's test (z14_order): zeta 1 or 4, lambda exactly +1 or -1, each pair once
+            if ((G.izeta[q] == 1 || G.izeta[q] == 4) && (G.lam[q] == 1.0 || G.lam[q] == -1.0))
+                seen |= 1 << ((G.izeta[q] == 4 ? 2 : 0) + (G.lam[q] < 0.0 ? 1 : 0));
+            else
+                seen |= 1 << 4;
         }
         p4 = p4 && tanh && allp4;
         z1n2 = z1n2 && tanh && allz1 && G.n_ang <= 2;
+        z14 = z14 && tanh && G.n_ang == 4 && seen == 15;
     }
-    return z1n2 ? PT_ACSF_TANH_Z1N2 : (p4 ? PT_ACSF_TANH_P4 : PT_ACSF_GENERIC);
+    return z1n2 ? PT_ACSF_TANH_Z1N2 : (z14 ? PT_ACSF_TANH_Z14 : (p4 ? PT_ACSF_TANH_P4 : PT_ACSF_GENERIC));
 }

@@ -368,11 +368,12 @@ def test_cfg1_subset_and_fd(dev):
 
 
 @pytest.mark.parametrize("cutoff", ["tanh", "tanhu", "cos", "exp", "poly2"])
-@pytest.mark.parametrize("zetas,kind", [((1.0, 2.0), "G3"), ((2.0, 3.0), "G3"), ((2.5, 1.0), "G3"), ((1.0, 4.0), "G9")])
+@pytest.mark.parametrize("zetas,kind", [((1.0, 2.0), "G3"), ((2.0, 3.0), "G3"), ((2.5, 1.0), "G3"), ((1.0, 4.0), "G9"),
+                                        ((4.0, 1.0), "G3")])
 def test_nnp_cutoff_and_zeta_paths_vs_oracle(dev, cutoff, zetas, kind):
     """Every cutoff type and angular-term pattern the descriptor kernels specialise on
-    (zeta all 1 / in {1, 2, 4} / other integers / non-integer via pow; G3 and G9), FP64 and
-    mixed, through energy and -grad E against the oracle."""
+    (zeta all 1 / in {1, 2, 4} / exactly {1, 4} x lambda +-1 / other integers / non-integer
+    via pow; G3 and G9), FP64 and mixed, through energy and -grad E against the oracle."""
     from oracle.bridge import oracle_nnp
     from pantea_amd.bench_models import make_nnp
     from pantea_amd.descriptors import ACSF, G2, G3, G9, CutoffFunction, NeighborElements
@@ -409,7 +410,7 @@ def test_wrong_kernel_kind_fails_loudly(dev):
     s = _water("cfg0")
     acsf = teacher_acsf("O")
     tab = acsf.device_table(s.element_map, dev)
-    assert tab.kind == _lib.PT_ACSF_TANH_P4
+    assert tab.kind == _lib.PT_ACSF_TANH_Z14
     nl = NeighborList(acsf.r_cutoff, s.element_map.n_types).build(s.positions, s.atom_types, s.box_lengths,
                                                                    s.type_counts())
     t_o = s.element_map.element_to_atom_type["O"]
@@ -418,8 +419,28 @@ def test_wrong_kernel_kind_fails_loudly(dev):
     assert np.isfinite(good).all()
     keep = tab.kind
     try:
+        # the wider classes cover this table and agree with the specialised kernel
+        for other in (_lib.PT_ACSF_TANH_P4, _lib.PT_ACSF_GENERIC):
+            tab.kind = other
+            alt = acsf_forward(nl, tab, _lib.PT_F64, cnt, center_begin=b).cpu().numpy()
+            assert _rel(alt, good) < 1e-13
         tab.kind = _lib.PT_ACSF_TANH_Z1N2   # this table has 4 terms with zeta = 4: not covered
         bad = acsf_forward(nl, tab, _lib.PT_F64, cnt, center_begin=b).cpu().numpy()
     finally:
         tab.kind = keep
     assert np.isnan(bad).all()
+    # and a zeta {1, 2} table is outside the Z14 class
+    from pantea_amd.descriptors import ACSF, G2, G3, CutoffFunction, NeighborElements
+
+    cfn = CutoffFunction.from_type("tanh", 9.0)
+    p4 = ACSF("O", ((G2(cfn, 0.0, 0.02), NeighborElements("H")),),
+              tuple((G3(cfn, eta=0.01, zeta=z, lambda0=lam), NeighborElements("H", "H"))
+                    for z in (1.0, 2.0) for lam in (1.0, -1.0)))
+    tab2 = p4.device_table(s.element_map, dev)
+    assert tab2.kind == _lib.PT_ACSF_TANH_P4
+    tab2.kind = _lib.PT_ACSF_TANH_Z14
+    nl2 = NeighborList(p4.r_cutoff, s.element_map.n_types).build(s.positions, s.atom_types, s.box_lengths,
+                                                                  s.type_counts())
+    b2, cnt2 = nl2.type_block(t_o)
+    bad2 = acsf_forward(nl2, tab2, _lib.PT_F64, cnt2, center_begin=b2).cpu().numpy()
+    assert np.isnan(bad2).all()

@@ -111,11 +111,22 @@ def test_table_kind_classification():
     def kind(acsf):
         return lib.pt_sf_table_kind(ctypes.byref(compile_table(acsf.rows(emap))))
 
-    assert kind(teacher_acsf("O")) == _lib.PT_ACSF_TANH_P4          # zeta in {1, 4}, tanh
+    assert kind(teacher_acsf("O")) == _lib.PT_ACSF_TANH_Z14         # zeta {1, 4} x lambda {+1, -1}, tanh
     assert kind(student_acsf("H")) == _lib.PT_ACSF_TANH_Z1N2        # zeta = 1, 2 terms, tanh
-    assert kind(teacher_acsf("O", cutoff="tanhu")) == _lib.PT_ACSF_TANH_P4
+    assert kind(teacher_acsf("O", cutoff="tanhu")) == _lib.PT_ACSF_TANH_Z14
     assert kind(teacher_acsf("O", cutoff="cos")) == _lib.PT_ACSF_GENERIC
     cfn = CutoffFunction.from_type("tanh", 6.0)
+
+    def angular(terms):
+        return ACSF("H", ((G2(cfn, 0.0, 0.1), NeighborElements("H")),),
+                    tuple((G3(cfn, eta=eta, zeta=z, lambda0=lam), NeighborElements("H", "O"))
+                          for eta in (0.01, 0.1) for z, lam in terms))
+
+    z14 = ((1.0, 1.0), (1.0, -1.0), (4.0, 1.0), (4.0, -1.0))
+    assert kind(angular(z14)) == _lib.PT_ACSF_TANH_Z14
+    assert kind(angular(z14[::-1])) == _lib.PT_ACSF_TANH_Z14           # any order of the four terms
+    assert kind(angular(z14[:3])) == _lib.PT_ACSF_TANH_P4              # three of them
+    assert kind(angular(((1.0, 1.0), (2.0, -1.0), (4.0, 1.0), (4.0, -1.0)))) == _lib.PT_ACSF_TANH_P4
     z3 = ACSF("H", ((G2(cfn, 0.0, 0.1), NeighborElements("H")),),
               ((G3(cfn, eta=0.1, zeta=3.0, lambda0=1.0), NeighborElements("H", "H")),))
     assert kind(z3) == _lib.PT_ACSF_GENERIC

@@ -9,8 +9,9 @@ structure.  Legs, all on the same structure and protocol:
 
   surrogate (headline): the trained student (34 ACSF, MLP 34-16-16-1, mixed
       precision) drives; the FP64 teacher judges every 20 steps; a force RMSE
-      above 5 meV/Bohr rewinds to the last good state and the FP64 teacher
-      drives the next 200 steps.
+      above 5 meV/Bohr rewinds to the last good state and the mixed-precision
+      teacher drives the next 200 steps, judged the same way (a failure there
+      hands the window to the FP64 teacher).
   teacher mixed: descriptors + adjoint FP32, MLP split-BF16 on MFMA, FP64
       energy sum, force gather and integrator.
   teacher FP64: the plain FP64 GPU path.
@@ -175,8 +176,8 @@ def run_surrogate(teacher, student, s, steps, warmup, args):
 
     sur = SurrogateMD(teacher, student, s, seed=2023, judge_every=args.judge_every,
                       threshold=args.threshold_mev / units.TO_MILLI_ELECTRON_VOLT,
-                      fallback_steps=args.fallback_steps, skin=args.skin, thermostat=_thermostat(args),
-                      judge_atoms=args.judge_atoms or None)
+                      fallback_steps=args.fallback_steps, fallback_precision=args.fallback_precision,
+                      skin=args.skin, thermostat=_thermostat(args), judge_atoms=args.judge_atoms or None)
     sur.run(warmup)
     _sync()
     before = sur.stats.as_dict()
@@ -188,13 +189,18 @@ def run_surrogate(teacher, student, s, steps, warmup, args):
     dt = time.perf_counter() - t0
     timers, sur.md.timers = sur.md.timers, None
     after = sur.stats.as_dict()
-    delta = {k: after[k] - before[k] for k in ("steps", "student_steps", "fallback_steps", "discarded_steps",
-                                                "judges", "failures")}
+    delta = {k: after[k] - before[k] for k in ("steps", "student_steps", "fallback_steps", "reference_steps",
+                                                "discarded_steps", "judges", "failures", "fallback_failures")}
     r = np.asarray(sur.stats.judge_rmse[n_rmse:]) * units.TO_MILLI_ELECTRON_VOLT
-    delta["fallback_rate"] = delta["failures"] / delta["judges"] if delta["judges"] else 0.0
+    who = np.asarray(sur.stats.judge_driver[n_rmse:])
+    delta["fallback_precision"] = args.fallback_precision
     delta["teacher_share"] = delta["fallback_steps"] / max(delta["steps"], 1)
-    delta["judge_rmse_mev_per_bohr_median"] = float(np.median(r)) if r.size else None
-    delta["judge_rmse_mev_per_bohr_min"] = float(np.min(r)) if r.size else None
+    rs, rf = r[who == "student"], r[who == "fallback"]
+    delta["student_judges"] = int(rs.size)
+    delta["student_judge_rmse_mev_per_bohr_median"] = float(np.median(rs)) if rs.size else None
+    delta["student_judge_rmse_mev_per_bohr_min"] = float(np.min(rs)) if rs.size else None
+    delta["fallback_judges"] = int(rf.size)
+    delta["fallback_judge_rmse_mev_per_bohr_max"] = float(np.max(rf)) if rf.size else None
     return dt, timers, delta
 
 
@@ -218,6 +224,8 @@ def main() -> int:
     ap.add_argument("--tau-fs", type=float, default=0.5, help="Berendsen time constant (0: NVE)")
     ap.add_argument("--judge-every", type=int, default=20)
     ap.add_argument("--fallback-steps", type=int, default=200)
+    ap.add_argument("--fallback-precision", default="mixed", choices=("mixed", "float64"),
+                    help="precision of the fallback teacher (below the judge's it is judged too)")
     ap.add_argument("--threshold-mev", type=float, default=5.0)
     ap.add_argument("--judge-atoms", type=int, default=0,
                     help="judge on this many sampled atoms (their neighbourhoods only); 0: every atom")
@@ -359,9 +367,11 @@ def main() -> int:
 
     thermo = f"Berendsen 300 K tau {args.tau_fs} fs" if args.tau_fs > 0 else "NVE"
     judged = f"{args.judge_atoms} sampled atoms" if args.judge_atoms else "all atoms"
+    fb = ("mixed-precision teacher steps (judged the same way; failing -> FP64 teacher)"
+          if args.fallback_precision == "mixed" else "FP64 teacher steps")
     head = ("surrogate: student 34 ACSF 34-16-16-1 mixed drives, FP64 teacher judges every "
             f"{args.judge_every} steps on {judged}, > {args.threshold_mev} meV/Bohr -> rewind + "
-            f"{args.fallback_steps} FP64 teacher steps") if sur is not None else "mixed-precision teacher (no trained student found)"
+            f"{args.fallback_steps} {fb}") if sur is not None else "mixed-precision teacher (no trained student found)"
     out = {
         "metric": "atom-steps/s, 98k-atom water NNP-MD: mixed+surrogate vs FP64; force RMSE",
         "value": value,
@@ -373,7 +383,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+bf16x3(mfma)+f64-accumulate; f64 judge/fallback",
+        "dtype": "f32+bf16x3(mfma)+f64-accumulate; f64 judge",
         "data": "synthetic water (seeded random sequential addition), random-init teacher weights, "
                 "student trained on the GPU on teacher MD data",
         "config": {"workload": f"water NNP-MD, {n} atoms ({n // 3} H2O), box {args.box} Bohr, rc 12 Bohr, "
@@ -390,7 +400,8 @@ def main() -> int:
         "energy_error_mev_per_atom": e_err,
         "force_rmse_mev_per_bohr": f_rmse,
         "accuracy_note": "energy_error / force_rmse: mixed teacher vs FP64 teacher on the start structure; "
-                         "surrogate.judge_rmse_*: student vs FP64 teacher at every judge",
+                         "surrogate.*_judge_rmse_*: the driver (student / mixed fallback) vs the FP64 "
+                         "teacher at its judges",
         "force_rms_mev_per_bohr": f_rms,
         "phase_ms": {k: v[0] for k, v in phases.items()},
         "teacher_mixed_phase_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in phases_m.items()},

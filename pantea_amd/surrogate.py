@@ -8,6 +8,13 @@ RMSE exceeds ``threshold`` the run rewinds to the last state that passed and
 the teacher drives the next ``fallback_steps`` steps; then the student takes
 over again from the state the teacher reached.
 
+The fallback teacher may run at a lower precision than the judge (e.g. the
+mixed-precision teacher under an FP64 judge).  It is then judged like the
+student: at every ``judge_every`` steps and at the end of its window, and a
+failure rewinds to the last state that passed and hands the window to the
+judge's own model, which needs no judging.  A fallback at the judge's
+precision is the judge's model and is never judged.
+
 A sampled judge evaluates the teacher only where the sampled atoms' full
 forces need it: F_k = -sum_i dE_i/dr_k involves the centres i within the
 cutoff of k, so descriptors, MLP and adjoint run on the closed
@@ -38,11 +45,14 @@ from .units import units
 class SurrogateStats:
     steps: int = 0                 # net trajectory length (steps kept)
     student_steps: int = 0         # steps driven by the student (kept or discarded)
-    fallback_steps: int = 0        # steps driven by the teacher
-    discarded_steps: int = 0       # student steps thrown away by rewinds
+    fallback_steps: int = 0        # kept steps driven by the teacher (either precision)
+    reference_steps: int = 0       # of those, steps driven by the judge's own model
+    discarded_steps: int = 0       # steps thrown away by rewinds
     judges: int = 0
-    failures: int = 0
+    failures: int = 0              # failed judges (student or judged fallback)
+    fallback_failures: int = 0     # of those, failures of the judged fallback
     judge_rmse: list = field(default_factory=list)   # Ha/Bohr, one per judge
+    judge_driver: list = field(default_factory=list)  # "student" or "fallback", one per judge
     seconds: float = 0.0
 
     @property
@@ -59,7 +69,9 @@ class SurrogateStats:
         mev = units.TO_MILLI_ELECTRON_VOLT
         r = np.asarray(self.judge_rmse, dtype=np.float64) * mev
         return {"steps": self.steps, "student_steps": self.student_steps, "fallback_steps": self.fallback_steps,
+                "reference_steps": self.reference_steps,
                 "discarded_steps": self.discarded_steps, "judges": self.judges, "failures": self.failures,
+                "fallback_failures": self.fallback_failures,
                 "fallback_rate": self.fallback_rate, "teacher_share": self.teacher_share,
                 "judge_rmse_mev_per_bohr_median": float(np.median(r)) if r.size else None,
                 "judge_rmse_mev_per_bohr_max": float(np.max(r)) if r.size else None,
@@ -92,6 +104,8 @@ class SurrogateMD:
         self.f_student.label = f"student-{student_precision}"
         self.f_judge.label = f"teacher-{judge_precision}"
         self.f_fallback.label = f"teacher-{fallback_precision}"
+        # a fallback below the judge's precision is judged; the judge's own model is not
+        self.judged_fallback = fallback_precision != judge_precision
         # the list may have grown: forces of the driver at the current positions
         md.compute(md.force, "full")
         self.dt = float(time_step)
@@ -177,15 +191,37 @@ class SurrogateMD:
         self.good_step = self.stats.steps
         self.mode = "student"
 
-    def _to_fallback(self) -> None:
+    def _to_fallback(self, reference: bool = False) -> None:
+        """Rewind to the last state that passed and hand the next window to the
+        fallback teacher (or, ``reference``, to the judge's own model)."""
         md = self.md
-        self.stats.discarded_steps += self.stats.steps - self.good_step
+        discarded = self.stats.steps - self.good_step
+        self.stats.discarded_steps += discarded
+        if self.mode == "fallback":   # the fallback's own steps since its last pass
+            self.stats.fallback_steps -= discarded
         md.restore_state(self.good)
         self.stats.steps = self.good_step
-        md.use(self.f_fallback)
+        reference = reference or not self.judged_fallback
+        md.use(self.f_judge if reference else self.f_fallback)
         md.compute(md.force, "full")
-        self.mode = "fallback"
+        self.mode = "reference" if reference else "fallback"
         self.fallback_left = self.fallback_len
+
+    def _judge_step(self) -> bool:
+        """Judge the driver at the current positions; on a pass the state becomes the rewind point."""
+        st = self.stats
+        rmse = self.judge()
+        st.judges += 1
+        st.judge_rmse.append(rmse)
+        st.judge_driver.append(self.mode)
+        if rmse <= self.threshold:
+            self.good = self.md.save_state()
+            self.good_step = st.steps
+            return True
+        st.failures += 1
+        if self.mode == "fallback":
+            st.fallback_failures += 1
+        return False
 
     def run(self, n_steps: int) -> SurrogateStats:
         """Advance the kept trajectory by ``n_steps`` steps."""
@@ -198,23 +234,25 @@ class SurrogateMD:
         while st.steps < target:
             md.step(self.dt, self.integrator, "full", self.thermostat)
             st.steps += 1
-            if self.mode == "fallback":
-                st.fallback_steps += 1
+            if self.mode == "student":
+                st.student_steps += 1
+                if st.steps - self.good_step >= self.judge_every and not self._judge_step():
+                    self._to_fallback()
+                continue
+            st.fallback_steps += 1
+            if self.mode == "reference":
+                st.reference_steps += 1
                 self.fallback_left -= 1
                 if self.fallback_left == 0:
                     self._to_student()
                 continue
-            st.student_steps += 1
-            if st.steps - self.good_step >= self.judge_every:
-                rmse = self.judge()
-                st.judges += 1
-                st.judge_rmse.append(rmse)
-                if rmse <= self.threshold:
-                    self.good = md.save_state()
-                    self.good_step = st.steps
-                else:
-                    st.failures += 1
-                    self._to_fallback()
+            # judged fallback: at the judge interval and before handing back to the student
+            self.fallback_left -= 1
+            due = st.steps - self.good_step >= self.judge_every or self.fallback_left == 0
+            if due and not self._judge_step():
+                self._to_fallback(reference=True)
+            elif self.fallback_left == 0:
+                self._to_student()
         torch.cuda.synchronize()
         st.seconds += time.perf_counter() - t0
         return st

@@ -220,6 +220,57 @@ def test_surrogate_failed_judge_rewinds_to_teacher_trajectory():
     assert float((sur.positions - plain.positions).abs().max()) < 1e-9
 
 
+def test_surrogate_mixed_fallback_is_judged_and_drives():
+    """A mixed-precision fallback under the FP64 judge: the untrained student is rejected at
+    every judge, the mixed teacher passes its own judges and drives the kept trajectory,
+    which is then the plain mixed-teacher MD; the FP64 model never drives."""
+    from pantea_amd.bench_models import make_nnp, student_acsf
+    from pantea_amd.simulation import MDSimulator, System
+    from pantea_amd.surrogate import SurrogateMD
+    from pantea_amd.units import units
+
+    s, teacher = _cfg0()
+    student = make_nnp(student_acsf, (16, 16), seed=99, precision="mixed")
+    student.fit_scaler([s])
+    thr = 1.0 / units.TO_MILLI_ELECTRON_VOLT   # 1 meV/Bohr: far above mixed-vs-FP64, far below the student
+    sur = SurrogateMD(teacher, student, s, seed=8, judge_every=4, threshold=thr, fallback_steps=10,
+                      fallback_precision="mixed", skin=1.0)
+    st = sur.run(25)
+    assert st.steps == 25 and st.fallback_steps == 25 and st.reference_steps == 0
+    assert st.fallback_failures == 0
+    # student rejected at steps 4, 14, 24; the mixed windows judged at 4, 8, 10 / 14, 18, 20 / 24
+    assert st.failures == 3 and st.discarded_steps == 12
+    assert st.judges == 10
+    plain = System.from_structure(s, teacher, temperature=300.0, seed=8, force_mode="full", precision="mixed",
+                                  skin=1.0)
+    MDSimulator(0.5 * units.FROM_FEMTO_SECOND, integrator="physical").run(plain, 25)
+    assert float((sur.positions - plain.positions).abs().max()) < 1e-9
+
+
+def test_surrogate_failed_mixed_fallback_escalates_to_judge_model():
+    """threshold 0: the mixed fallback fails its first judge too, the run rewinds and the FP64
+    model drives; the kept trajectory is the FP64 teacher's."""
+    from pantea_amd.bench_models import make_nnp, student_acsf
+    from pantea_amd.simulation import MDSimulator, System
+    from pantea_amd.surrogate import SurrogateMD
+    from pantea_amd.units import units
+
+    s, teacher = _cfg0()
+    student = make_nnp(student_acsf, (16, 16), seed=99, precision="mixed")
+    student.fit_scaler([s])
+    sur = SurrogateMD(teacher, student, s, seed=8, judge_every=4, threshold=0.0, fallback_steps=10,
+                      fallback_precision="mixed", skin=1.0)
+    st = sur.run(25)
+    # per window: 4 student steps rejected, 4 mixed steps rejected, 10 FP64 steps kept
+    assert st.steps == 25 and st.fallback_steps == st.reference_steps == 25
+    assert st.judges == st.failures == 6 and st.fallback_failures == 3
+    assert st.discarded_steps == 24
+    plain = System.from_structure(s, teacher, temperature=300.0, seed=8, force_mode="full", precision="float64",
+                                  skin=1.0)
+    MDSimulator(0.5 * units.FROM_FEMTO_SECOND, integrator="physical").run(plain, 25)
+    assert float((sur.positions - plain.positions).abs().max()) < 1e-9
+
+
 def test_sampled_judge_forces_equal_full_evaluation():
     """The sampled judge evaluates the teacher on the sampled atoms' neighbourhoods only; its
     forces on those atoms equal the full evaluation's (same kernels, same FP64 sums)."""

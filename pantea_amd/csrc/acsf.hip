@@ -262,11 +262,11 @@ struct Stage {
     R* W;          // ADJ, FP64: the group's dE/dG weights (read by broadcast)
 };
 
-// FP64 adjoint: a compact record (x, y, z, r) + (fc, dfc) and no slot array
-// (1/r is recomputed per pair, edge positions by a second pass over the row),
-// which fits three waves per SIMD in LDS.
+// Adjoint: a compact record (x, y, z, r) + (fc, dfc) and no slot array (1/r
+// is recomputed per pair, edge positions by a second pass over the row): the
+// LDS per wave then allows three waves per SIMD in FP64 and four in FP32.
 template <typename R, int MODE>
-__host__ __device__ constexpr bool compact_adj() { return MODE == MODE_ADJ && sizeof(R) == 8; }
+__host__ __device__ constexpr bool compact_adj() { return MODE == MODE_ADJ; }
 
 template <typename R, int MODE>
 __host__ __device__ constexpr size_t stage_bytes_per_nbr() {

@@ -229,13 +229,20 @@ def test_surrogate_mixed_fallback_is_judged_and_drives():
     from pantea_amd.surrogate import SurrogateMD
     from pantea_amd.units import units
 
+    from pantea_amd.simulation import BrendsenThermostat
+
     s, teacher = _cfg0()
     student = make_nnp(student_acsf, (16, 16), seed=99, precision="mixed")
     student.fit_scaler([s])
-    thr = 1.0 / units.TO_MILLI_ELECTRON_VOLT   # 1 meV/Bohr: far above mixed-vs-FP64, far below the student
+    # 5 meV/Bohr: the mixed teacher stays within ~1 of FP64 at 300 K, the untrained
+    # student is off by ~1e4
+    thr = 5.0 / units.TO_MILLI_ELECTRON_VOLT
+    thermo = BrendsenThermostat(300.0, 0.5 * units.FROM_FEMTO_SECOND)
     sur = SurrogateMD(teacher, student, s, seed=8, judge_every=4, threshold=thr, fallback_steps=10,
-                      fallback_precision="mixed", skin=1.0)
+                      fallback_precision="mixed", skin=1.0, thermostat=thermo)
     st = sur.run(25)
+    print("judge RMSE (meV/Bohr):", [round(r * units.TO_MILLI_ELECTRON_VOLT, 3) for r in st.judge_rmse],
+          st.judge_driver)
     assert st.steps == 25 and st.fallback_steps == 25 and st.reference_steps == 0
     assert st.fallback_failures == 0
     # student rejected at steps 4, 14, 24; the mixed windows judged at 4, 8, 10 / 14, 18, 20 / 24
@@ -243,8 +250,12 @@ def test_surrogate_mixed_fallback_is_judged_and_drives():
     assert st.judges == 10
     plain = System.from_structure(s, teacher, temperature=300.0, seed=8, force_mode="full", precision="mixed",
                                   skin=1.0)
-    MDSimulator(0.5 * units.FROM_FEMTO_SECOND, integrator="physical").run(plain, 25)
-    assert float((sur.positions - plain.positions).abs().max()) < 1e-9
+    MDSimulator(0.5 * units.FROM_FEMTO_SECOND, thermostat=thermo, integrator="physical").run(plain, 25)
+    # The rewinds rebuild the neighbour list at other steps than the plain run, which
+    # reorders atoms and so the FP32 sums: the two mixed trajectories agree to FP32
+    # rounding grown over 25 steps (~1e-5 Bohr), not bit for bit. A student-driven
+    # step would move atoms by orders of magnitude more.
+    assert float((sur.positions - plain.positions).abs().max()) < 1e-3
 
 
 def test_surrogate_failed_mixed_fallback_escalates_to_judge_model():

@@ -252,8 +252,8 @@ int pt_mlp_energy_grad(int mode, int n_atoms, const void *G, int ldg, int g_prec
 /* ------------------------------------------------------------------------
  * (c') Training of an atomic MLP (the student surrogate).  No counterpart in
  * the reference runs (its trainer, potentials/nnp/gradient_descent.py:
- * 157-233, is broken); the loss follows BASELINE: w_E ((E - E*)/N)^2 +
- * w_F / (3N) sum |F - F*|^2 per structure.
+ * 157-233, is broken); the loss is the energy MSE per atom plus a weighted
+ * force MSE, w_E ((E - E*)/N)^2 + w_F / (3N) sum |F - F*|^2 per structure.
  * pt_mlp_train_grad accumulates into grad (FP64, layout of the unpacked
  * params) the gradient  sum_i gy[i] dy_i/dtheta + tseed d(ydot_i)/dtheta,
  * y = MLP(scaled G), ydot = grad_G y . T (T: descriptor tangent).  With

@@ -665,7 +665,7 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
     const int lane = wave_lane();
     const int wv = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
-    const int c = uniform(blockIdx.x * wpb + wv);
+    const int c = uniform(xcd_block() * wpb + wv);
     if (c >= a.n_centers) return;
     const int cap = a.cap;
     Stage<R> S = carve<R, MODE>(smem + (size_t)wv * stage_bytes<R, MODE>(cap), cap);

@@ -142,6 +142,18 @@ __device__ __forceinline__ int wave_lane() { return threadIdx.x & (kWave - 1); }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Workgroups are dispatched to the 8 XCDs of the chip round-robin in
+// blockIdx order, and each XCD has its own L2.  This maps the launch so that
+// each XCD takes a contiguous eighth of the work items instead of every
+// eighth one, which keeps spatially close items (shared neighbours) in one
+// L2.  A bijection on [0, gridDim.x); a tail of fewer than 8 is unchanged.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int full = nb / kXcds * kXcds;
+    return b < full ? (b % kXcds) * (full / kXcds) + b / kXcds : b;
+}
+
 // Exclusive prefix count of a ballot mask below this lane.
 __device__ __forceinline__ int lanes_below(unsigned long long mask) {
     unsigned lo = (unsigned)mask, hi = (unsigned)(mask >> 32);

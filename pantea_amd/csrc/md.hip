@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void k_force_gather(int n, const long long* __
                                                       const E* __restrict__ g,
                                                       double* __restrict__ f) {
     const int lane = wave_lane();
-    const int i = uniform(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const int i = uniform(xcd_block() * 4 + (threadIdx.x >> 6));
     if (i >= n) return;
     const long long b = row_off[i], e = row_off[i + 1];
     double fx = 0, fy = 0, fz = 0;

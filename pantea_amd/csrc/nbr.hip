@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256) void k_nbr_cell(const double* __restrict__ pos
     __shared__ int s_nr;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x >> 6;
-    const int cell = blockIdx.x;
+    const int cell = xcd_block();
 
     if (threadIdx.x == 0) {
         int acc = 0;
@@ -641,7 +641,7 @@ __global__ __launch_bounds__(256) void k_compact_rev(int n, const long long* __r
                                                      int* __restrict__ rev) {
     static_assert(kBucket == 16 && kStageW % kBucket == 0, "four int4 per bucket");
     const int lane = threadIdx.x & (kWave - 1);
-    const int i = uniform(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const int i = uniform(xcd_block() * 4 + (threadIdx.x >> 6));
     if (i >= n) return;
     const long long oi = row_off[i];
     const int ci = counts[i];

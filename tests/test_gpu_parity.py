@@ -204,6 +204,30 @@ def test_acsf_fp64_fp32_vs_oracle(dev, cutoff):
         assert _rel(g32, ref) < 2e-5
 
 
+@pytest.mark.parametrize("cutoff", ["tanh", "cos"])
+def test_acsf_four_eta_groups_vs_oracle(dev, cutoff):
+    """Four eta per angular group spanning 0.002 .. 0.45 (ratios 3, 1.25, 60), both
+    cutoff classes, FP64 and FP32 against the oracle."""
+    from oracle.bridge import sf_rows
+    from oracle import pyoracle
+    from pantea_amd.descriptors import ACSF, G2, G3, CutoffFunction, NeighborElements
+
+    s = _water("cfg0")
+    pos = s.positions.cpu().numpy()
+    emap = s.element_map.element_to_atom_type
+    cfn = CutoffFunction.from_type(cutoff, 10.0)
+    acsf = ACSF("O", ((G2(cfn, 0.0, 0.02), NeighborElements("H")),),
+                tuple((G3(cfn, eta=eta, zeta=z, lambda0=lam), NeighborElements(j, k))
+                      for (j, k) in (("H", "H"), ("H", "O")) for eta in (0.002, 0.006, 0.0075, 0.45)
+                      for z in (1.0, 4.0) for lam in (1.0, -1.0)))
+    idx = s.select_host("O")
+    ref = pyoracle.acsf(pos, s.atom_types_host, s.box_lengths, sf_rows(acsf, emap), center_pos=pos[idx],
+                       self_idx=idx.astype(np.int32))
+    assert np.max(np.abs(ref[:, -8:])) > 0   # the eta = 0.45 terms are not all underflowed
+    assert _rel(acsf(s).cpu().numpy(), ref) < 1e-10
+    assert _rel(acsf(s, precision="float32").cpu().numpy(), ref) < 2e-5
+
+
 def test_acsf_grad_vs_oracle(dev):
     from oracle.bridge import sf_rows
     from oracle import pyoracle

@@ -1,8 +1,8 @@
 // (b) ACSF descriptors, their adjoint and their tangent on gfx950.
 //
 // One wavefront (64 lanes) per centre.  The centre's neighbour row is staged
-// in LDS (relative vectors R_ij = r_i - r_j under the minimum image, r_ij,
-// 1/r_ij, r_ij^2, type and row slot), compacted to r_ij <= rc_max and
+// in LDS (relative vectors R_ij = r_i - r_j under the minimum image with r_ij
+// or 1/r_ij, and per mode the cutoff values), compacted to r_ij <= rc_max and
 // r_ij > 0 (atoms/neighbor.py:102-107).  Rows arrive sorted by type, so each
 // neighbour element is a contiguous LDS segment.
 //
@@ -251,35 +251,36 @@ constexpr int kQueue = 128;
 
 template <typename R>
 struct Stage {
-    V4<R>* P;      // (x, y, z, 1/r) of R_ij = r_i - r_j
-    V4<R>* Q;      // ADJ, TAN: (r, r^2, fc, dfc/dr) -- cutoff of the current group
+    V4<R>* P;      // (x, y, z, 1/r) of R_ij = r_i - r_j; ADJ: (x, y, z, r)
+    V4<R>* Q;      // TAN: (r, r^2, fc, dfc/dr) -- cutoff of the current group
     V2<R>* Q2;     // FWD: (r, fc)
-    V2<R>* QF;     // compact FP64 ADJ: (fc, dfc); r is P.w
+    V2<R>* QF;     // ADJ: (fc, dfc)
     R* D;          // [3][cap] TAN: delta_i - delta_j
     double* DA;    // [3][cap] ADJ: per-edge accumulators (FP64: fast LDS atomics)
-    int* slot;     // ADJ: row slot of each staged neighbour
+    unsigned long long* keep;   // ADJ: the staging ballots, one per 64 row entries
     unsigned* queue;
     R* W;          // ADJ, FP64: the group's dE/dG weights (read by broadcast)
 };
 
-// Adjoint: a compact record (x, y, z, r) + (fc, dfc) and no slot array (1/r
-// is recomputed per pair, edge positions by a second pass over the row): the
-// LDS per wave then allows three waves per SIMD in FP64 and four in FP32.
-template <typename R, int MODE>
-__host__ __device__ constexpr bool compact_adj() { return MODE == MODE_ADJ; }
+// The adjoint keeps a compact record (x, y, z, r) + (fc, dfc) and no
+// per-neighbour row slot (1/r is recomputed per pair, edge positions come
+// back from the staging ballots): the LDS per wave then allows three waves
+// per SIMD in FP64 and four in FP32.
 
 template <typename R, int MODE>
 __host__ __device__ constexpr size_t stage_bytes_per_nbr() {
-    if (compact_adj<R, MODE>()) return sizeof(double) * 3 + sizeof(R) * 6;
-    return sizeof(double) * (MODE == MODE_ADJ ? 3 : 0) +
-           sizeof(R) * (MODE == MODE_FWD ? 6 : 8 + (MODE == MODE_TAN ? 3 : 0)) + (MODE == MODE_ADJ ? sizeof(int) : 0);
+    return MODE == MODE_ADJ ? sizeof(double) * 3 + sizeof(R) * 6
+                            : sizeof(R) * (MODE == MODE_FWD ? 6 : 11);
 }
 
-// cap is a multiple of 4; the per-wave area is a multiple of 32 bytes
+// cap is a multiple of 4 and >= the row length; the per-wave area is a
+// multiple of 32 bytes
 template <typename R, int MODE>
 __host__ __device__ constexpr size_t stage_bytes(int cap) {
     return ((size_t)cap * stage_bytes_per_nbr<R, MODE>() + kQueue * sizeof(unsigned) +
-            (MODE == MODE_ADJ ? PT_AG_ETA * PT_AG_ANG * sizeof(R) : 0) + 31) / 32 * 32;
+            (MODE == MODE_ADJ ? PT_AG_ETA * PT_AG_ANG * sizeof(R) +
+                                    (size_t)((cap + kWave - 1) / kWave) * sizeof(unsigned long long)
+                              : 0) + 31) / 32 * 32;
 }
 
 template <typename R, int MODE>
@@ -289,7 +290,7 @@ __device__ __forceinline__ Stage<R> carve(char* base, int cap) {
     if (MODE == MODE_ADJ) { s.DA = reinterpret_cast<double*>(p); p += 3 * cap * sizeof(double); } else { s.DA = nullptr; }
     s.P = reinterpret_cast<V4<R>*>(p); p += cap * sizeof(V4<R>);
     s.Q = nullptr; s.Q2 = nullptr; s.QF = nullptr;
-    if (compact_adj<R, MODE>()) {
+    if ((MODE == MODE_ADJ)) {
         s.QF = reinterpret_cast<V2<R>*>(p); p += cap * sizeof(V2<R>);
     } else if (MODE == MODE_FWD) {
         s.Q2 = reinterpret_cast<V2<R>*>(p); p += cap * sizeof(V2<R>);
@@ -297,14 +298,15 @@ __device__ __forceinline__ Stage<R> carve(char* base, int cap) {
         s.Q = reinterpret_cast<V4<R>*>(p); p += cap * sizeof(V4<R>);
     }
     if (MODE == MODE_TAN) { s.D = reinterpret_cast<R*>(p); p += 3 * cap * sizeof(R); } else { s.D = nullptr; }
-    if (MODE == MODE_ADJ && !compact_adj<R, MODE>()) {
-        s.slot = reinterpret_cast<int*>(p); p += cap * sizeof(int);
-    } else {
-        s.slot = nullptr;
-    }
     s.queue = reinterpret_cast<unsigned*>(p);
     p += kQueue * sizeof(unsigned);
-    s.W = MODE == MODE_ADJ ? reinterpret_cast<R*>(p) : nullptr;
+    s.W = nullptr;
+    s.keep = nullptr;
+    if (MODE == MODE_ADJ) {
+        s.W = reinterpret_cast<R*>(p);
+        p += PT_AG_ETA * PT_AG_ANG * sizeof(R);
+        s.keep = reinterpret_cast<unsigned long long*>(p);
+    }
     return s;
 }
 
@@ -431,7 +433,7 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
         const int sb = pass == 0 ? X.bj : X.bk, sn = pass == 0 ? X.nj : X.nk;
         for (int s = sb + lane; s < sb + sn; s += kWave) {
             R f, df;
-            if constexpr (compact_adj<R, MODE>()) {
+            if constexpr ((MODE == MODE_ADJ)) {
                 cutoff_k<R, CUT>(X.cut, X.rc, X.irc, X.cpre, X.tanh_pre, S.P[s].w, f, df);
                 V2<R> fv;
                 fv.x = f; fv.y = df;
@@ -453,7 +455,7 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
         const V4<R> pa = S.P[ia], pb = S.P[ib];
         V4<R> qa, qb;   // (r, r^2, fc, dfc)
         R ira, irb;     // 1 / r
-        if constexpr (compact_adj<R, MODE>()) {
+        if constexpr ((MODE == MODE_ADJ)) {
             const V2<R> ua = S.QF[ia], ub = S.QF[ib];
             qa.x = pa.w; qa.y = pa.w * pa.w; qa.z = ua.x; qa.w = ua.y;
             qb.x = pb.w; qb.y = pb.w * pb.w; qb.z = ub.x; qb.w = ub.y;
@@ -467,7 +469,7 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
             qa = S.Q[ia];
             qb = S.Q[ib];
         }
-        if constexpr (!compact_adj<R, MODE>()) {
+        if constexpr ((MODE != MODE_ADJ)) {
             ira = pa.w;
             irb = pb.w;
         }
@@ -702,14 +704,15 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
             keep = (r <= rcmax) && (r > 0.0);
         }
         const unsigned long long bal = __ballot(keep);
+        if (MODE == MODE_ADJ && lane == 0) S.keep[t0 / kWave] = bal;
         if (keep) {
             const int s = m + lanes_below(bal);
             V4<R> pv, qv;
             pv.x = (R)dx; pv.y = (R)dy; pv.z = (R)dz;
-            pv.w = compact_adj<R, MODE>() ? (R)r : (R)(1.0 / r);
+            pv.w = (MODE == MODE_ADJ) ? (R)r : (R)(1.0 / r);
             qv.x = (R)r; qv.y = (R)r * (R)r; qv.z = R(0); qv.w = R(0);
             S.P[s] = pv;
-            if constexpr (compact_adj<R, MODE>()) {
+            if constexpr ((MODE == MODE_ADJ)) {
                 // (fc, dfc) are written per group
             } else if constexpr (MODE == MODE_FWD) {
                 V2<R> uv;
@@ -718,7 +721,6 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
             } else {
                 S.Q[s] = qv;
             }
-            if (MODE == MODE_ADJ && !compact_adj<R, MODE>()) S.slot[s] = t;
             if (MODE == MODE_ADJ) { S.DA[s] = 0.0; S.DA[cap + s] = 0.0; S.DA[2 * cap + s] = 0.0; }
             if (MODE == MODE_TAN) {
                 double ex = dix, ey = diy, ez = diz;
@@ -765,7 +767,7 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
         }
         for (int s = b + lane; s < e; s += kWave) {
             R r;
-            if constexpr (compact_adj<R, MODE>()) r = S.P[s].w;
+            if constexpr ((MODE == MODE_ADJ)) r = S.P[s].w;
             else if constexpr (MODE == MODE_FWD) r = S.Q2[s].x;
             else r = S.Q[s].x;
             R f, df;
@@ -787,7 +789,7 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
                 }
             }
             if (MODE == MODE_ADJ) {
-                const R sc = dsum * (compact_adj<R, MODE>() ? M<R>::rcp(pv.w) : pv.w);
+                const R sc = dsum * ((MODE == MODE_ADJ) ? M<R>::rcp(pv.w) : pv.w);
                 const R gx = sc * pv.x, gy = sc * pv.y, gz = sc * pv.z;
                 S.DA[s] -= (double)gx; S.DA[cap + s] -= (double)gy; S.DA[2 * cap + s] -= (double)gz;
             }
@@ -920,35 +922,18 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
     if (MODE == MODE_ADJ) {
         if (a.edge_grad) {
             R* eg = (R*)a.edge_grad;
-            if constexpr (compact_adj<R, MODE>()) {
-                // the staged order again: same test as the staging pass
-                int ms = 0;
-                for (int t0 = 0; t0 < len; t0 += kWave) {
-                    const int t = t0 + lane;
-                    bool keep = false;
-                    if (t < len) {
-                        const double4 pj = reinterpret_cast<const double4*>(a.pos4)[a.nbr[off + t]];
-                        double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                        const double r = exact_dist(dx, dy, dz, a.box);
-                        keep = (r <= rcmax) && (r > 0.0);
-                    }
-                    const unsigned long long bal = __ballot(keep);
-                    if (keep) {
-                        const int s = ms + lanes_below(bal);
-                        const long long e = off + t;
-                        eg[3 * e + 0] = bad ? nan : (R)S.DA[s];
-                        eg[3 * e + 1] = bad ? nan : (R)S.DA[cap + s];
-                        eg[3 * e + 2] = bad ? nan : (R)S.DA[2 * cap + s];
-                    }
-                    ms += __popcll(bal);
-                }
-            } else {
-                for (int s = lane; s < m; s += kWave) {
-                    const long long e = off + S.slot[s];
+            // the staging ballots give each kept row entry its staged slot
+            int ms = 0;
+            for (int t0 = 0; t0 < len; t0 += kWave) {
+                const unsigned long long bal = S.keep[t0 / kWave];
+                if ((bal >> lane) & 1ull) {
+                    const int s = ms + lanes_below(bal);
+                    const long long e = off + t0 + lane;
                     eg[3 * e + 0] = bad ? nan : (R)S.DA[s];
                     eg[3 * e + 1] = bad ? nan : (R)S.DA[cap + s];
                     eg[3 * e + 2] = bad ? nan : (R)S.DA[2 * cap + s];
                 }
+                ms += __popcll(bal);
             }
         }
         if (a.f_self) {

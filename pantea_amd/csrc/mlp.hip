@@ -140,6 +140,26 @@ __device__ __forceinline__ void activate(int a, T z, T& y, T& dy) {
     y = z; dy = T(1);
 }
 
+// Activations whose derivative is a function of the output y: the inference
+// kernel then keeps only y and turns it into dy/dz in place once the next
+// layer has read it.
+__host__ __device__ constexpr bool act_from_output(int a) {
+    return a == PT_ACT_IDENTITY || a == PT_ACT_TANH || a == PT_ACT_LOGISTIC || a == PT_ACT_SOFTPLUS ||
+           a == PT_ACT_RELU || a == PT_ACT_EXP;
+}
+
+template <typename T>
+__device__ __forceinline__ T dact_from_output(int a, T y) {
+    switch (a) {
+    case PT_ACT_TANH: return T(1) - y * y;
+    case PT_ACT_LOGISTIC: return y * (T(1) - y);
+    case PT_ACT_SOFTPLUS: return -expm1(-y);          // logistic(z) = 1 - e^{-y}
+    case PT_ACT_RELU: return y > T(0) ? T(1) : T(0);   // y = max(z, 0)
+    case PT_ACT_EXP: return -y;                        // y = e^{-z}
+    }
+    return T(1);
+}
+
 // y, dy/dz and d2y/dz2 (the training kernel differentiates the input gradient)
 template <typename T>
 __device__ __forceinline__ void activate2(int a, T z, T& y, T& dy, T& ddy) {
@@ -235,6 +255,7 @@ struct MlpArgs {
     void* dEdG;
     int stride;           // LDS row stride (elements)
     int nbuf;             // LDS buffers per wave
+    int inplace;          // every hidden activation has act_from_output
 };
 
 // C = A(16 x K, LDS row-major, stride) * B, accumulating one 16-column tile.
@@ -305,16 +326,24 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
     const int wpb = blockDim.x >> 6;
     const int stride = a.stride;
     T* base = reinterpret_cast<T*>(smem) + (size_t)wv * ((size_t)a.nbuf * 16 * stride + 16);
-    T* X[2] = {base, base + 16 * stride};         // ping-pong activations
+    // 16-row buffers: the input in slot 0, hidden output h in slot ys(h)
+    // (slot 1, then slot 0 once the input is dead, then 2, 3, ...).  With
+    // a.inplace the derivative of layer h replaces its output in place after
+    // layer h + 1 has read it (max(2, H) slots); otherwise it has a slot of
+    // its own after those.
+    const int H = s.n_hidden;
+    const int ny = H > 2 ? H : 2;
+    auto ys = [](int h) { return h == 0 ? 1 : (h == 1 ? 0 : h); };
     T* D[kMaxHidden];                              // activation derivatives
-    for (int h = 0; h < kMaxHidden; ++h) D[h] = base + (2 + h) * 16 * stride;
-    T* gout = base + (2 + s.n_hidden) * 16 * stride;  // 16 output derivatives
+    for (int h = 0; h < kMaxHidden; ++h)
+        D[h] = base + (size_t)(a.inplace ? ys(h) : ny + h) * 16 * stride;
+    T* gout = base + (size_t)a.nbuf * 16 * stride;  // 16 output derivatives
 
     const int row = l & 15, grp = l >> 4;
     const int ntiles = (a.n_atoms + 15) / 16;
     for (int tile = blockIdx.x * wpb + wv; tile < ntiles; tile += gridDim.x * wpb) {
         const int a0 = tile * 16;
-        // 1. scaled input -> X[0] (zero padded)
+        // 1. scaled input -> slot 0 (zero padded)
         for (int e = l; e < 16 * s.in_p; e += kWave) {
             const int r = e / s.in_p, k = e % s.in_p;
             const int atom = a0 + r;
@@ -324,21 +353,23 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
                                          : (double)((const float*)a.G)[(size_t)atom * a.ldg + k];
                 v = (T)((g - a.scale[k]) * a.scale[s.in_dim + k] + a.scale_add);
             }
-            X[0][r * stride + k] = v;
+            base[r * stride + k] = v;
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
 
         // 2. hidden layers
-        int cur = 0, kin = s.in_p;
-        for (int h = 0; h < s.n_hidden; ++h) {
+        int kin = s.in_p;
+        T* xin = base;
+        for (int h = 0; h < H; ++h) {
             const int wp = s.wp[h];
+            T* yout = base + (size_t)ys(h) * 16 * stride;
             for (int n0 = 0; n0 < wp; n0 += 16) {
                 T acc[4] = {T(0), T(0), T(0), T(0)};
                 if (MODE == PT_MLP_F64)
-                    gemm_tile<MODE>(X[cur], stride, kin, a.packed, s.W64[h], 0, wp, false, n0, acc);
+                    gemm_tile<MODE>(xin, stride, kin, a.packed, s.W64[h], 0, wp, false, n0, acc);
                 else
-                    gemm_tile<MODE>(X[cur], stride, kin, a.packed, s.WTh[h], s.WTl[h], kin, true, n0, acc);
+                    gemm_tile<MODE>(xin, stride, kin, a.packed, s.WTh[h], s.WTl[h], kin, true, n0, acc);
                 const int col = n0 + row;
                 const T bias = (MODE == PT_MLP_F64) ? (T)((const double*)(a.packed + s.b64[h]))[col]
                                                     : (T)((const float*)(a.packed + s.b32[h]))[col];
@@ -348,15 +379,27 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
                     T y, dy;
                     activate<T>(s.act[h], acc[i] + bias, y, dy);
                     if (col >= s.w[h]) { y = T(0); dy = T(0); }
-                    X[cur ^ 1][r * stride + col] = y;
-                    D[h][r * stride + col] = dy;
+                    yout[r * stride + col] = y;
+                    if (!a.inplace) D[h][r * stride + col] = dy;
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            cur ^= 1;
+            if (a.inplace && h > 0) {
+                // layer h has read the output of layer h - 1: turn it into its derivative
+                const int wq = s.wp[h - 1];
+                for (int e = l; e < 16 * wq; e += kWave) {
+                    const int r = e / wq, k = e % wq;
+                    T* p = D[h - 1] + r * stride + k;
+                    *p = k < s.w[h - 1] ? dact_from_output<T>(s.act[h - 1], *p) : T(0);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
+            xin = yout;
             kin = wp;
         }
+        T* xlast = xin;
 
         // 3. output layer (width 1): 4 lanes per atom
         {
@@ -366,7 +409,7 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
                                                 : (T)((const float*)(a.packed + s.bout32))[0];
             const int r = l >> 2, part = l & 3;
             T sum = T(0);
-            for (int k = part; k < kin; k += 4) sum += X[cur][r * stride + k] * wout[k];
+            for (int k = part; k < kin; k += 4) sum += xlast[r * stride + k] * wout[k];
             sum += __shfl_xor(sum, 1, kWave);
             sum += __shfl_xor(sum, 2, kWave);
             T y, dy;
@@ -387,7 +430,10 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
             const int h = s.n_hidden - 1;
             for (int e = l; e < 16 * s.wp[h]; e += kWave) {
                 const int r = e / s.wp[h], k = e % s.wp[h];
-                D[h][r * stride + k] *= gout[r] * wout[k];
+                T* p = D[h] + r * stride + k;
+                // in place: the slot still holds the last hidden output
+                const T dy = !a.inplace ? *p : (k < s.w[h] ? dact_from_output<T>(s.act[h], *p) : T(0));
+                *p = dy * gout[r] * wout[k];
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -747,7 +793,10 @@ template <int MODE>
 int launch_mlp(MlpArgs a, hipStream_t st) {
     typedef typename MT<MODE>::T T;
     a.stride = a.s.wmax + (sizeof(T) == 8 ? 1 : 4);
-    a.nbuf = 2 + a.s.n_hidden;
+    a.inplace = 1;
+    for (int h = 0; h < a.s.n_hidden; ++h) a.inplace = a.inplace && act_from_output(a.s.act[h]);
+    const int ny = a.s.n_hidden > 2 ? a.s.n_hidden : 2;
+    a.nbuf = ny + (a.inplace ? 0 : a.s.n_hidden);
     // nbuf activation buffers of 16 rows plus 16 output derivatives
     const size_t per_wave = sizeof(T) * ((size_t)a.nbuf * 16 * a.stride + 16);
     const size_t budget = lds_limit();

@@ -254,15 +254,29 @@ def test_acsf_grad_vs_oracle(dev):
 @pytest.mark.parametrize("mode,tol", [("float64", 1e-12), ("bf16x3", 2e-4), ("bf16", 3e-2)])
 @pytest.mark.parametrize("shape", [(52, (25, 25)), (52, (64, 64)), (34, (16, 16)), (3, (5, 5)), (70, (100, 40, 8))])
 def test_mlp_vs_oracle(dev, mode, tol, shape):
+    n_in, hidden = shape
+    _check_mlp(dev, mode, tol, n_in, hidden, ["tanh", "softplus", "logistic"][: len(hidden)])
+
+
+@pytest.mark.parametrize("mode,tol", [("float64", 1e-12), ("bf16x3", 2e-4)])
+@pytest.mark.parametrize("acts", [("relu", "exp", "identity"), ("gaussian", "cos", "harmonic"), ("tanh", "gaussian")])
+def test_mlp_activation_paths_vs_oracle(dev, mode, tol, acts):
+    """Activations whose derivative follows from the output (the kernel turns the stored
+    output into the derivative in place) and ones whose derivative does not (it keeps a
+    derivative buffer per layer), alone and mixed."""
+    hidden = (32, 24, 16)[: len(acts)]
+    _check_mlp(dev, mode, tol, 40, hidden, list(acts), weight_scale=0.4)
+
+
+def _check_mlp(dev, mode, tol, n_in, hidden, acts, weight_scale=1.0):
     from oracle import pyoracle
     from pantea_amd import _lib
     from pantea_amd.models import MLP_MODES, DeviceMLP, NeuralNetworkModel, NormalInitializer
 
-    n_in, hidden = shape
-    acts = ["tanh", "softplus", "logistic"][: len(hidden)]
     model = NeuralNetworkModel(tuple(zip(hidden, acts)), (1, "identity"), kernel_initializer=NormalInitializer())
     params = model.init(11, n_in)
     for k in params:
+        params[k]["kernel"] = params[k]["kernel"] * weight_scale
         params[k]["bias"] = np.random.default_rng(3).normal(size=params[k]["bias"].shape) * 0.1
     n = 203
     rng = np.random.default_rng(5)
@@ -277,7 +291,7 @@ def test_mlp_vs_oracle(dev, mode, tol, shape):
     mlp.energy_grad(MLP_MODES[mode], n, Gd, n_in, _lib.PT_F64 if mode == "float64" else _lib.PT_F32, e, dE)
     torch.cuda.synchronize()
     layers = [(params[f"layers_{2*i}"]["kernel"], params[f"layers_{2*i}"]["bias"], a)
-              for i, a in enumerate(acts + ["identity"])]
+              for i, a in enumerate(list(acts) + ["identity"])]
     x = (G - sub) * mul + add
     y, g = pyoracle.mlp_forward_backward(layers, x)
     g = g * mul

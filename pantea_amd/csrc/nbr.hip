@@ -638,8 +638,13 @@ __global__ __launch_bounds__(256) void k_compact_rev(int n, const long long* __r
                                                      const int* __restrict__ counts,
                                                      const int* __restrict__ stage,
                                                      const int* __restrict__ heads, int* __restrict__ nbr,
-                                                     int* __restrict__ rev) {
+                                                     int* __restrict__ rev,
+                                                     const long long* __restrict__ total_and_max,
+                                                     long long cap) {
     static_assert(kBucket == 16 && kStageW % kBucket == 0, "four int4 per bucket");
+    // launched before the host has read the totals: nothing to do unless the
+    // list fits the arrays and every row was staged whole
+    if (total_and_max[0] > cap || total_and_max[1] > kStageW) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int i = uniform(xcd_block() * 4 + (threadIdx.x >> 6));
     if (i >= n) return;
@@ -871,9 +876,25 @@ extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int*
     hipLaunchKernelGGL(k_scan_down, dim3(nsb), dim3(256), 0, stream, n, counts, psum, tot_max,
                        row_off);
     if (int rcode = launch_check("row offsets")) return rcode;
-    long long host_tm[2];
-    PT_CHECK_HIP(hipMemcpyAsync(host_tm, tot_max, sizeof(host_tm), hipMemcpyDeviceToHost, stream));
-    PT_CHECK_HIP(hipStreamSynchronize(stream));
+    // The totals go to pinned host memory behind an event, and the compaction
+    // is queued before the host waits for them (it checks them itself), so the
+    // GPU does not idle through the round trip.
+    thread_local long long* host_tm = nullptr;
+    if (!host_tm) PT_CHECK_HIP(hipHostMalloc((void**)&host_tm, 2 * sizeof(long long), hipHostMallocDefault));
+    hipEvent_t ev;
+    PT_CHECK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    PT_CHECK_HIP(hipMemcpyAsync(host_tm, tot_max, 2 * sizeof(long long), hipMemcpyDeviceToHost, stream));
+    PT_CHECK_HIP(hipEventRecord(ev, stream));
+    if (nbr) {
+        hipLaunchKernelGGL(k_compact_rev, dim3(nwb), dim3(256), 0, stream, n, (const long long*)row_off,
+                           (const int*)counts, (const int*)stage, (const int*)heads, nbr, rev,
+                           (const long long*)tot_max, cap);
+    }
+    const int lrc = launch_check("k_compact_rev");
+    const hipError_t werr = hipEventSynchronize(ev);
+    (void)hipEventDestroy(ev);
+    PT_CHECK_HIP(werr);
+    if (lrc) return lrc;
     *n_pairs = host_tm[0];
     *max_row = (int)host_tm[1];
     if (host_tm[0] > cap) {
@@ -881,11 +902,7 @@ extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int*
         return PT_ERR_CAPACITY;
     }
     PT_REQUIRE(nbr != nullptr || host_tm[0] == 0, "pt_nbr_build: nbr is null");
-    if (host_tm[1] <= kStageW) {
-        hipLaunchKernelGGL(k_compact_rev, dim3(nwb), dim3(256), 0, stream, n, (const long long*)row_off,
-                           (const int*)counts, (const int*)stage, (const int*)heads, nbr, rev);
-        return launch_check("k_compact_rev");
-    }
+    if (host_tm[1] <= kStageW) return PT_OK;
     hipLaunchKernelGGL(k_nbr_cell<PASS_FILL>, dim3(g.ncell), dim3(256), 0, stream, pos4,
                        (const float4*)pos4f, (const unsigned long long*)bmax, cell_start,
                        g, ba, n_types, thr, counts, row_off, nbr, (int*)nullptr);

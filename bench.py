@@ -150,8 +150,11 @@ def _thermostat(args):
     return BrendsenThermostat(300.0, args.tau_fs * units.FROM_FEMTO_SECOND) if args.tau_fs > 0 else None
 
 
-def run_md(pot, s, precision, steps, warmup, args):
-    """Plain MD of one potential; returns (seconds, per-phase HIP-event timers, system)."""
+def run_md(pot, s, precision, steps, warmup, args, instrument=False):
+    """Plain MD of one potential; returns (seconds, per-phase HIP-event timers or None, system).
+
+    Timed runs are not instrumented (the phase events cost host time and
+    GPU idle); an instrumented run gives the per-phase breakdown."""
     from pantea_amd.simulation import MDSimulator, System
     from pantea_amd.units import units
 
@@ -161,7 +164,7 @@ def run_md(pot, s, precision, steps, warmup, args):
     md.run(sysm, warmup)
     _sync()
     eng = sysm._engine()
-    eng.timers = {}
+    eng.timers = {} if instrument else None
     t0 = time.perf_counter()
     md.run(sysm, steps)
     _sync()
@@ -170,7 +173,7 @@ def run_md(pot, s, precision, steps, warmup, args):
     return dt, timers, sysm
 
 
-def run_surrogate(teacher, student, s, steps, warmup, args):
+def run_surrogate(teacher, student, s, steps, warmup, args, instrument=False):
     from pantea_amd.surrogate import SurrogateMD
     from pantea_amd.units import units
 
@@ -182,7 +185,7 @@ def run_surrogate(teacher, student, s, steps, warmup, args):
     _sync()
     before = sur.stats.as_dict()
     n_rmse = len(sur.stats.judge_rmse)
-    sur.md.timers = {}
+    sur.md.timers = {} if instrument else None
     t0 = time.perf_counter()
     sur.run(steps)
     _sync()
@@ -278,9 +281,12 @@ def main() -> int:
         _log(f"[bench] student vs FP64 teacher on the start structure: {stud_acc}")
 
     # teacher legs
-    dt_m, timers_m, _ = run_md(teacher, s, "mixed", args.teacher_steps, args.warmup, args)
+    dt_m, _, _ = run_md(teacher, s, "mixed", args.teacher_steps, args.warmup, args)
     mixed_value = n * args.teacher_steps / dt_m
     _log(f"[bench] teacher mixed MD: {1e3 * dt_m / args.teacher_steps:.3f} ms/step, {mixed_value:.4g} atom-steps/s")
+    # per-phase breakdown from an instrumented repeat
+    phase_steps_m = min(args.teacher_steps, 50)
+    _, timers_m, _ = run_md(teacher, s, "mixed", phase_steps_m, 2, args, instrument=True)
     phases_m = phase_table(timers_m)
     fp64_value = None
     if not args.no_fp64:
@@ -299,14 +305,18 @@ def main() -> int:
     # headline leg
     sur = None
     if student is not None:
-        dt_s, timers_s, sur = run_surrogate(teacher, student, s, args.steps, args.warmup, args)
+        dt_s, _, sur = run_surrogate(teacher, student, s, args.steps, args.warmup, args)
         value = n * args.steps / dt_s
         ms_per_step = 1e3 * dt_s / args.steps
-        phases = phase_table(timers_s)
         steps = args.steps
         _log(f"[bench] surrogate MD: {ms_per_step:.3f} ms/step, {value:.4g} atom-steps/s, {sur}")
+        # phases and the roofline kernel's launch time from an instrumented repeat
+        phase_steps = min(args.steps, 200)
+        _, timers_s, _ = run_surrogate(teacher, student, s, phase_steps, args.warmup, args, instrument=True)
+        phases = phase_table(timers_s)
     else:
         value, ms_per_step, phases, steps = mixed_value, 1e3 * dt_m / args.teacher_steps, phases_m, args.teacher_steps
+        phase_steps = phase_steps_m
 
     # roofline of the kernel with the most time in the headline run
     n_nb, n_trip = neighbour_stats(s, teacher.r_cutoff)
@@ -341,8 +351,8 @@ def main() -> int:
                 f"algorithmic ops per atom {('2.5 x ' if mode else '')}((15 + 6 g2) n_nb + (26 + 5 g3) n_trip), "
                 f"g2 = 8, g3 = {6 if is_student else 12} per neighbour pair type, transcendental = 1, FMA = 2; "
                 f"measured n_nb={n_nb:.1f}, n_trip={n_trip:.0f} per atom; a launch covers the O or the H centres "
-                f"(n/2 on average); average launch time from HIP events over {groups} launch pairs of the "
-                f"headline run; peak = published MI355X {'FP64' if fp64 else 'FP32'} vector; traffic = HBM bytes "
+                f"(n/2 on average); average launch time from HIP events over {groups} launch pairs of an "
+                f"instrumented {phase_steps}-step repeat of the headline run; peak = published MI355X {'FP64' if fp64 else 'FP32'} vector; traffic = HBM bytes "
                 "per launch from the FETCH_SIZE/WRITE_SIZE profiles, copy-kernel calibrated",
     }
     _log(f"[bench] roofline: {roofline['kernel']} {achieved:.3g} TFLOP/s = {roofline['frac']:.3f} of peak")
@@ -403,8 +413,10 @@ def main() -> int:
                          "surrogate.*_judge_rmse_*: the driver (student / mixed fallback) vs the FP64 "
                          "teacher at its judges",
         "force_rms_mev_per_bohr": f_rms,
-        "phase_ms": {k: v[0] for k, v in phases.items()},
-        "teacher_mixed_phase_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in phases_m.items()},
+        "phase_ms_per_step": {k: v[0] / phase_steps for k, v in phases.items()},
+        "phase_note": f"HIP events of instrumented repeats ({phase_steps} headline steps, {phase_steps_m} mixed "
+                      "teacher steps); the timed runs carry no events",
+        "teacher_mixed_phase_ms_per_step": {k: v[0] / phase_steps_m for k, v in phases_m.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -687,6 +687,13 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
         dix = a.delta[3 * i]; diy = a.delta[3 * i + 1]; diz = a.delta[3 * i + 2];
     }
 
+    // segment counts are needed up to one past the largest type the table uses
+    int tmax = 0;
+    for (int g = 0; g < uniform(tab->n_rg); ++g) tmax = max(tmax, uniform(tab->rg[g].elem));
+    for (int g = 0; g < uniform(tab->n_ag); ++g)
+        tmax = max(tmax, max(uniform(tab->ag[g].ej), uniform(tab->ag[g].ek)));
+    const int nseg = min(tmax + 1, kSegTypes);
+
     // ---- stage the row ----------------------------------------------------
     int m = 0;
     int seg = 0;   // lane q: staged neighbours with type < q
@@ -735,8 +742,7 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
         }
         m += __popcll(bal);
         const int tk = (int)tj;
-#pragma unroll
-        for (int q = 1; q <= kSegTypes; ++q) {
+        for (int q = 1; q <= nseg; ++q) {
             const int cq = __popcll(__ballot(keep && tk < q));
             seg += lane == q ? cq : 0;
         }

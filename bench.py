@@ -332,11 +332,16 @@ def main() -> int:
     launch_ops = (adj_ops if mode else fwd_ops) * n / 2.0
     achieved = launch_ops / (launch_ms * 1e-3) / 1e12 if launch_ms > 0 else 0.0
     peak = PEAK_FP64_TFLOPS if fp64 else PEAK_FP32_TFLOPS
-    # the launch's kernel: table class (pt_sf_table_kind) and the four-wave FP64 forward variant
+    # the launch's kernel: table class (pt_sf_table_kind) and the four-/eight-wave forward variants
     pot_of = student if is_student else teacher
     kind = pot_of.atomic_potentials["O"].descriptor.device_table(s.element_map, s.device).kind
-    w4 = "_w4" if (fp64 and mode == 0 and kind != 0) else ""
-    kname = f"k_acsf{w4}<{'double' if fp64 else 'float'}, {mode}, {kind}>"
+    if mode == 0 and fp64 and kind != 0:
+        variant = "_w4"
+    elif mode == 0 and not fp64 and kind in (2, 3):
+        variant = "_w8"
+    else:
+        variant = ""
+    kname = f"k_acsf{variant}<{'double' if fp64 else 'float'}, {mode}, {kind}>"
     roofline = {
         "bound": "mfma",
         "achieved": achieved,

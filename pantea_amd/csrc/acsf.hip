@@ -247,7 +247,10 @@ template <typename R> struct alignas(2 * sizeof(R)) V2 { R x, y; };
 
 // Live-pair queue of one wavefront: (a | b << 16) indices of pairs that
 // pass the r_jk screen.  Batches of 64 are taken as soon as they are full.
-constexpr int kQueue = 128;
+// The forward screens two diagonals per lane and step, so its queue holds
+// up to 63 + 128 entries before it is drained.
+template <int MODE>
+__host__ __device__ constexpr int queue_len() { return MODE == 0 ? 192 : 128; }
 
 template <typename R>
 struct Stage {
@@ -277,7 +280,7 @@ __host__ __device__ constexpr size_t stage_bytes_per_nbr() {
 // multiple of 32 bytes
 template <typename R, int MODE>
 __host__ __device__ constexpr size_t stage_bytes(int cap) {
-    return ((size_t)cap * stage_bytes_per_nbr<R, MODE>() + kQueue * sizeof(unsigned) +
+    return ((size_t)cap * stage_bytes_per_nbr<R, MODE>() + queue_len<MODE>() * sizeof(unsigned) +
             (MODE == MODE_ADJ ? PT_AG_ETA * PT_AG_ANG * sizeof(R) +
                                     (size_t)((cap + kWave - 1) / kWave) * sizeof(unsigned long long)
                               : 0) + 31) / 32 * 32;
@@ -299,7 +302,7 @@ __device__ __forceinline__ Stage<R> carve(char* base, int cap) {
     }
     if (MODE == MODE_TAN) { s.D = reinterpret_cast<R*>(p); p += 3 * cap * sizeof(R); } else { s.D = nullptr; }
     s.queue = reinterpret_cast<unsigned*>(p);
-    p += kQueue * sizeof(unsigned);
+    p += queue_len<MODE>() * sizeof(unsigned);
     s.W = nullptr;
     s.keep = nullptr;
     if (MODE == MODE_ADJ) {
@@ -598,41 +601,91 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
     // test is the cutoff's own (it is 0 beyond rc)
     const R rc2 = X.g3 ? X.rc * X.rc * R(1.0001) : R(INFINITY);
     const int sk = X.same ? 1 : 0;
+    auto screen = [&](int ia, int ib) {
+        const V4<R> pa = S.P[ia], pb = S.P[ib];
+        R jx = pa.x - pb.x, jy = pa.y - pb.y, jz = pa.z - pb.z;
+        if (X.mic) mic3(*X.box, jx, jy, jz);
+        const R r2 = jx * jx + jy * jy + jz * jz;
+        return (r2 > R(0)) & (r2 < rc2);
+    };
     int qn = 0, base = 0;
-    while (true) {
-        if (base < npairs) {
-            int k = q + sk + d;
-            k = k >= ncol ? k - ncol : k;
-            const int ia = brow + q, ib = bcol + k;
-            bool live = false;
-            if (base + lane < npairs && !(d == half_d && q >= half_q)) {
-                const V4<R> pa = S.P[ia], pb = S.P[ib];
-                R jx = pa.x - pb.x, jy = pa.y - pb.y, jz = pa.z - pb.z;
-                if (X.mic) mic3(*X.box, jx, jy, jz);
-                const R r2 = jx * jx + jy * jy + jz * jz;
-                live = (r2 > R(0)) & (r2 < rc2);
+    if constexpr (MODE == MODE_FWD) {
+        // Two diagonals per lane and step: lane (D, q) screens d = 2D and 2D + 1,
+        // which share the row slot and most of the index arithmetic.  (The
+        // adjoint keeps one diagonal per lane: two pairs of one row slot in a
+        // batch would collide on its LDS accumulators.)
+        const int hh = (h + 1) / 2;
+        const int npairs2 = nrow * hh;
+        d = lane / nrow;   // D
+        while (true) {
+            if (base < npairs2) {
+                const int d0 = 2 * d;
+                int k0 = q + sk + d0;
+                k0 = k0 >= ncol ? k0 - ncol : k0;
+                int k1 = k0 + 1;
+                k1 = k1 >= ncol ? k1 - ncol : k1;
+                const int ia = brow + q;
+                const bool in = base + lane < npairs2;
+                const bool v0 = in && !(d0 == half_d && q >= half_q);
+                const bool v1 = in && d0 + 1 < h && !(d0 + 1 == half_d && q >= half_q);
+                const bool live0 = v0 && screen(ia, bcol + k0);
+                const bool live1 = v1 && screen(ia, bcol + k1);
+                const unsigned long long b0 = __ballot(live0);
+                if (live0) S.queue[qn + lanes_below(b0)] = (unsigned)ia | ((unsigned)(bcol + k0) << 16);
+                qn += __popcll(b0);
+                const unsigned long long b1 = __ballot(live1);
+                if (live1) S.queue[qn + lanes_below(b1)] = (unsigned)ia | ((unsigned)(bcol + k1) << 16);
+                qn += __popcll(b1);
+                base += kWave;
+                q += r64;
+                d += d64;
+                if (q >= nrow) { q -= nrow; ++d; }
             }
-            const unsigned long long bal = __ballot(live);
-            if (live) S.queue[qn + lanes_below(bal)] = (unsigned)ia | ((unsigned)ib << 16);
-            qn += __popcll(bal);
-            base += kWave;
-            q += r64;
-            d += d64;
-            if (q >= nrow) { q -= nrow; ++d; }
+            const bool last = base >= npairs2;
+            while (qn >= kWave || (last && qn > 0)) {
+                wave_sync();
+                const int take = qn < kWave ? qn : kWave;
+                const unsigned e = lane < take ? S.queue[lane] : 0u;
+                const int rem = qn - take;
+                const unsigned mv0 = lane < rem ? S.queue[kWave + lane] : 0u;
+                const unsigned mv1 = lane + kWave < rem ? S.queue[2 * kWave + lane] : 0u;
+                wave_sync();
+                if (lane < rem) S.queue[lane] = mv0;
+                if (lane + kWave < rem) S.queue[kWave + lane] = mv1;
+                qn = rem;
+                if (lane < take) body((int)(e & 0xffffu), (int)(e >> 16));
+            }
+            if (last && qn == 0) break;
         }
-        const bool last = base >= npairs;
-        if (qn >= kWave || (last && qn > 0)) {
-            wave_sync();
-            const int take = qn < kWave ? qn : kWave;
-            const unsigned e = lane < take ? S.queue[lane] : 0u;
-            const int rem = qn - take;
-            const unsigned mv = lane < rem ? S.queue[kWave + lane] : 0u;
-            wave_sync();
-            if (lane < rem) S.queue[lane] = mv;
-            qn = rem;
-            if (lane < take) body((int)(e & 0xffffu), (int)(e >> 16));
+    } else {
+        while (true) {
+            if (base < npairs) {
+                int k = q + sk + d;
+                k = k >= ncol ? k - ncol : k;
+                const int ia = brow + q, ib = bcol + k;
+                const bool live = base + lane < npairs && !(d == half_d && q >= half_q) && screen(ia, ib);
+                const unsigned long long bal = __ballot(live);
+                if (live) S.queue[qn + lanes_below(bal)] = (unsigned)ia | ((unsigned)ib << 16);
+                qn += __popcll(bal);
+                base += kWave;
+                q += r64;
+                d += d64;
+                if (q >= nrow) { q -= nrow; ++d; }
+            }
+            const bool last = base >= npairs;
+            if (qn >= kWave || (last && qn > 0)) {
+                wave_sync();
+                const int take = qn < kWave ? qn : kWave;
+                const unsigned e = lane < take ? S.queue[lane] : 0u;
+                const int rem = qn - take;
+                const unsigned mv = lane < rem ? S.queue[kWave + lane] : 0u;
+                wave_sync();
+                if (lane < rem) S.queue[lane] = mv;
+                qn = rem;
+                if (lane < take) body((int)(e & 0xffffu), (int)(e >> 16));
+            }
+            if (last && qn == 0) break;
         }
-        if (last && qn == 0) break;
     }
     if (kPreOut && MODE != MODE_ADJ) {
 #pragma unroll
@@ -961,9 +1014,10 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
     }
 }
 
-// Two entry kernels over the same body: the second asks the compiler for
-// four resident waves per SIMD (<= 128 VGPRs); the FP64 forward of the
-// specialised table classes fits that without spills.
+// Entry kernels over the same body: the others ask the compiler for four
+// (<= 128 VGPRs) or eight (<= 64 VGPRs) resident waves per SIMD; the FP64
+// forwards of the specialised table classes and the FP32 forwards of the
+// zeta {1, 4} and zeta = 1 classes fit those without spills.
 template <typename R, int MODE, int KIND>
 __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
     extern __shared__ __align__(32) char smem[];
@@ -972,6 +1026,12 @@ __global__ __launch_bounds__(256) void k_acsf(KArgs a) {
 
 template <typename R, int MODE, int KIND>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_acsf_w4(KArgs a) {
+    extern __shared__ __align__(32) char smem[];
+    acsf_centre<R, MODE, KIND>(a, smem);
+}
+
+template <typename R, int MODE, int KIND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_acsf_w8(KArgs a) {
     extern __shared__ __align__(32) char smem[];
     acsf_centre<R, MODE, KIND>(a, smem);
 }
@@ -1003,6 +1063,8 @@ int launch(KArgs a, hipStream_t stream) {
     const int blocks = (a.n_centers + wpb - 1) / wpb;
     if constexpr (sizeof(R) == 8 && MODE == MODE_FWD && KIND != PT_ACSF_GENERIC)
         hipLaunchKernelGGL((k_acsf_w4<R, MODE, KIND>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, stream, a);
+    else if constexpr (sizeof(R) == 4 && MODE == MODE_FWD && (KIND == PT_ACSF_TANH_Z14 || KIND == PT_ACSF_TANH_Z1N2))
+        hipLaunchKernelGGL((k_acsf_w8<R, MODE, KIND>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, stream, a);
     else
         hipLaunchKernelGGL((k_acsf<R, MODE, KIND>), dim3(blocks), dim3(64 * wpb), wpb * per_wave, stream, a);
     return launch_check("k_acsf");

@@ -247,8 +247,8 @@ template <typename R> struct alignas(2 * sizeof(R)) V2 { R x, y; };
 
 // Live-pair queue of one wavefront: (a | b << 16) indices of pairs that
 // pass the r_jk screen.  Batches of 64 are taken as soon as they are full.
-// The forward screens two diagonals per lane and step, so its queue holds
-// up to 63 + 128 entries before it is drained.
+// Pairs are screened two diagonals per lane and step; the forward queues
+// both before it drains, so its queue holds up to 63 + 128 entries.
 template <int MODE>
 __host__ __device__ constexpr int queue_len() { return MODE == 0 ? 192 : 128; }
 
@@ -609,83 +609,54 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
         return (r2 > R(0)) & (r2 < rc2);
     };
     int qn = 0, base = 0;
-    if constexpr (MODE == MODE_FWD) {
-        // Two diagonals per lane and step: lane (D, q) screens d = 2D and 2D + 1,
-        // which share the row slot and most of the index arithmetic.  (The
-        // adjoint keeps one diagonal per lane: two pairs of one row slot in a
-        // batch would collide on its LDS accumulators.)
-        const int hh = (h + 1) / 2;
-        const int npairs2 = nrow * hh;
-        d = lane / nrow;   // D
-        while (true) {
-            if (base < npairs2) {
-                const int d0 = 2 * d;
-                int k0 = q + sk + d0;
-                k0 = k0 >= ncol ? k0 - ncol : k0;
-                int k1 = k0 + 1;
-                k1 = k1 >= ncol ? k1 - ncol : k1;
-                const int ia = brow + q;
-                const bool in = base + lane < npairs2;
-                const bool v0 = in && !(d0 == half_d && q >= half_q);
-                const bool v1 = in && d0 + 1 < h && !(d0 + 1 == half_d && q >= half_q);
-                const bool live0 = v0 && screen(ia, bcol + k0);
-                const bool live1 = v1 && screen(ia, bcol + k1);
-                const unsigned long long b0 = __ballot(live0);
-                if (live0) S.queue[qn + lanes_below(b0)] = (unsigned)ia | ((unsigned)(bcol + k0) << 16);
-                qn += __popcll(b0);
-                const unsigned long long b1 = __ballot(live1);
-                if (live1) S.queue[qn + lanes_below(b1)] = (unsigned)ia | ((unsigned)(bcol + k1) << 16);
-                qn += __popcll(b1);
-                base += kWave;
-                q += r64;
-                d += d64;
-                if (q >= nrow) { q -= nrow; ++d; }
-            }
-            const bool last = base >= npairs2;
-            while (qn >= kWave || (last && qn > 0)) {
-                wave_sync();
-                const int take = qn < kWave ? qn : kWave;
-                const unsigned e = lane < take ? S.queue[lane] : 0u;
-                const int rem = qn - take;
-                const unsigned mv0 = lane < rem ? S.queue[kWave + lane] : 0u;
-                const unsigned mv1 = lane + kWave < rem ? S.queue[2 * kWave + lane] : 0u;
-                wave_sync();
-                if (lane < rem) S.queue[lane] = mv0;
-                if (lane + kWave < rem) S.queue[kWave + lane] = mv1;
-                qn = rem;
-                if (lane < take) body((int)(e & 0xffffu), (int)(e >> 16));
-            }
-            if (last && qn == 0) break;
+    // one batch of up to 64 queued pairs through the body; the rest moves up
+    auto drain = [&]() {
+        wave_sync();
+        const int take = qn < kWave ? qn : kWave;
+        const unsigned e = lane < take ? S.queue[lane] : 0u;
+        const int rem = qn - take;
+        const unsigned mv0 = lane < rem ? S.queue[kWave + lane] : 0u;
+        const unsigned mv1 = lane + kWave < rem ? S.queue[2 * kWave + lane] : 0u;
+        wave_sync();
+        if (lane < rem) S.queue[lane] = mv0;
+        if (lane + kWave < rem) S.queue[kWave + lane] = mv1;
+        qn = rem;
+        if (lane < take) body((int)(e & 0xffffu), (int)(e >> 16));
+    };
+    // Two diagonals per lane and step: lane (D, q) screens d = 2D and 2D + 1,
+    // which share the row slot and most of the index arithmetic.  The forward
+    // queues both before draining (queue of 192); the adjoint drains between
+    // the two (queue of 128, so its LDS per wave stays put).
+    const int hh = (h + 1) / 2;
+    const int npairs2 = nrow * hh;
+    while (true) {
+        if (base < npairs2) {
+            const int d0 = 2 * d;
+            int k0 = q + sk + d0;
+            k0 = k0 >= ncol ? k0 - ncol : k0;
+            int k1 = k0 + 1;
+            k1 = k1 >= ncol ? k1 - ncol : k1;
+            const int ia = brow + q;
+            const bool in = base + lane < npairs2;
+            const bool v0 = in && !(d0 == half_d && q >= half_q);
+            const bool v1 = in && d0 + 1 < h && !(d0 + 1 == half_d && q >= half_q);
+            const bool live0 = v0 && screen(ia, bcol + k0);
+            const bool live1 = v1 && screen(ia, bcol + k1);
+            const unsigned long long b0 = __ballot(live0);
+            if (live0) S.queue[qn + lanes_below(b0)] = (unsigned)ia | ((unsigned)(bcol + k0) << 16);
+            qn += __popcll(b0);
+            if (MODE != MODE_FWD && qn >= kWave) drain();
+            const unsigned long long b1 = __ballot(live1);
+            if (live1) S.queue[qn + lanes_below(b1)] = (unsigned)ia | ((unsigned)(bcol + k1) << 16);
+            qn += __popcll(b1);
+            base += kWave;
+            q += r64;
+            d += d64;
+            if (q >= nrow) { q -= nrow; ++d; }
         }
-    } else {
-        while (true) {
-            if (base < npairs) {
-                int k = q + sk + d;
-                k = k >= ncol ? k - ncol : k;
-                const int ia = brow + q, ib = bcol + k;
-                const bool live = base + lane < npairs && !(d == half_d && q >= half_q) && screen(ia, ib);
-                const unsigned long long bal = __ballot(live);
-                if (live) S.queue[qn + lanes_below(bal)] = (unsigned)ia | ((unsigned)ib << 16);
-                qn += __popcll(bal);
-                base += kWave;
-                q += r64;
-                d += d64;
-                if (q >= nrow) { q -= nrow; ++d; }
-            }
-            const bool last = base >= npairs;
-            if (qn >= kWave || (last && qn > 0)) {
-                wave_sync();
-                const int take = qn < kWave ? qn : kWave;
-                const unsigned e = lane < take ? S.queue[lane] : 0u;
-                const int rem = qn - take;
-                const unsigned mv = lane < rem ? S.queue[kWave + lane] : 0u;
-                wave_sync();
-                if (lane < rem) S.queue[lane] = mv;
-                qn = rem;
-                if (lane < take) body((int)(e & 0xffffu), (int)(e >> 16));
-            }
-            if (last && qn == 0) break;
-        }
+        const bool last = base >= npairs2;
+        while (qn >= kWave || (last && qn > 0)) drain();
+        if (last && qn == 0) break;
     }
     if (kPreOut && MODE != MODE_ADJ) {
 #pragma unroll

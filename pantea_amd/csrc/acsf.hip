@@ -246,11 +246,10 @@ template <typename R> struct alignas(4 * sizeof(R)) V4 { R x, y, z, w; };
 template <typename R> struct alignas(2 * sizeof(R)) V2 { R x, y; };
 
 // Live-pair queue of one wavefront: (a | b << 16) indices of pairs that
-// pass the r_jk screen.  Batches of 64 are taken as soon as they are full.
-// Pairs are screened two diagonals per lane and step; the forward queues
-// both before it drains, so its queue holds up to 63 + 128 entries.
+// pass the r_jk screen; a batch of 64 is taken as soon as it is full, so it
+// never holds more than 63 + 64 entries.
 template <int MODE>
-__host__ __device__ constexpr int queue_len() { return MODE == 0 ? 192 : 128; }
+__host__ __device__ constexpr int queue_len() { return 128; }
 
 template <typename R>
 struct Stage {
@@ -623,32 +622,32 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
         qn = rem;
         if (lane < take) body((int)(e & 0xffffu), (int)(e >> 16));
     };
-    // Two diagonals per lane and step: lane (D, q) screens d = 2D and 2D + 1,
-    // which share the row slot and most of the index arithmetic.  The forward
-    // queues both before draining (queue of 192); the adjoint drains between
-    // the two (queue of 128, so its LDS per wave stays put).
-    const int hh = (h + 1) / 2;
+    // kDiag diagonals per lane and step: lane (D, q) screens d = kDiag D + u,
+    // u = 0 .. kDiag - 1, which share the row slot and the index arithmetic;
+    // the queue is drained whenever it holds a full batch, so it never holds
+    // more than 63 + 64 entries.  (Measured on gfx950: 4 beats 2 and 8 in
+    // FP32; the FP64 adjoint, short of registers, does best with 2.)
+    constexpr int kDiag = (sizeof(R) == 8 && MODE == MODE_ADJ) ? 2 : 4;
+    const int hh = (h + kDiag - 1) / kDiag;
     const int npairs2 = nrow * hh;
     while (true) {
         if (base < npairs2) {
-            const int d0 = 2 * d;
-            int k0 = q + sk + d0;
-            k0 = k0 >= ncol ? k0 - ncol : k0;
-            int k1 = k0 + 1;
-            k1 = k1 >= ncol ? k1 - ncol : k1;
             const int ia = brow + q;
             const bool in = base + lane < npairs2;
-            const bool v0 = in && !(d0 == half_d && q >= half_q);
-            const bool v1 = in && d0 + 1 < h && !(d0 + 1 == half_d && q >= half_q);
-            const bool live0 = v0 && screen(ia, bcol + k0);
-            const bool live1 = v1 && screen(ia, bcol + k1);
-            const unsigned long long b0 = __ballot(live0);
-            if (live0) S.queue[qn + lanes_below(b0)] = (unsigned)ia | ((unsigned)(bcol + k0) << 16);
-            qn += __popcll(b0);
-            if (MODE != MODE_FWD && qn >= kWave) drain();
-            const unsigned long long b1 = __ballot(live1);
-            if (live1) S.queue[qn + lanes_below(b1)] = (unsigned)ia | ((unsigned)(bcol + k1) << 16);
-            qn += __popcll(b1);
+            int dd = kDiag * d;
+            int k = q + sk + dd;
+            k = k >= ncol ? k - ncol : k;
+            for (int u = 0; u < kDiag; ++u) {
+                const bool v = in && dd < h && !(dd == half_d && q >= half_q);
+                const bool live = v && screen(ia, bcol + k);
+                const unsigned long long b = __ballot(live);
+                if (live) S.queue[qn + lanes_below(b)] = (unsigned)ia | ((unsigned)(bcol + k) << 16);
+                qn += __popcll(b);
+                if (qn >= kWave) drain();
+                ++dd;
+                ++k;
+                k = k >= ncol ? k - ncol : k;
+            }
             base += kWave;
             q += r64;
             d += d64;

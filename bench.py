@@ -8,8 +8,9 @@ MLP 52-64-64-1 tanh, weights ~ N(0, 1/fan_in), min-max scaler fitted on the
 structure.  Legs, all on the same structure and protocol:
 
   surrogate (headline): the trained student (34 ACSF, MLP 34-16-16-1, mixed
-      precision) drives; the FP64 teacher judges every 20 steps; a force RMSE
-      above 5 meV/Bohr rewinds to the last good state and the mixed-precision
+      precision) drives; the FP64 teacher judges every 20 steps (every 5
+      while the student is on probation after a failure); a force RMSE above
+      5 meV/Bohr rewinds to the last good state and the mixed-precision
       teacher drives the next 200 steps, judged the same way (a failure there
       hands the window to the FP64 teacher).
   teacher mixed: descriptors + adjoint FP32, MLP split-BF16 on MFMA, FP64
@@ -180,7 +181,8 @@ def run_surrogate(teacher, student, s, steps, warmup, args, instrument=False):
     sur = SurrogateMD(teacher, student, s, seed=2023, judge_every=args.judge_every,
                       threshold=args.threshold_mev / units.TO_MILLI_ELECTRON_VOLT,
                       fallback_steps=args.fallback_steps, fallback_precision=args.fallback_precision,
-                      skin=args.skin, thermostat=_thermostat(args), judge_atoms=args.judge_atoms or None)
+                      skin=args.skin, thermostat=_thermostat(args), judge_atoms=args.judge_atoms or None,
+                      probation_every=args.probation_every)
     sur.run(warmup)
     _sync()
     before = sur.stats.as_dict()
@@ -197,7 +199,8 @@ def run_surrogate(teacher, student, s, steps, warmup, args, instrument=False):
     r = np.asarray(sur.stats.judge_rmse[n_rmse:]) * units.TO_MILLI_ELECTRON_VOLT
     who = np.asarray(sur.stats.judge_driver[n_rmse:])
     delta["fallback_precision"] = args.fallback_precision
-    delta["teacher_share"] = delta["fallback_steps"] / max(delta["steps"], 1)
+    # share of the kept trajectory driven by the teacher (a rewind can reach back into the warm-up)
+    delta["teacher_share"] = after["teacher_share"]
     rs, rf = r[who == "student"], r[who == "fallback"]
     delta["student_judges"] = int(rs.size)
     delta["student_judge_rmse_mev_per_bohr_median"] = float(np.median(rs)) if rs.size else None
@@ -227,6 +230,8 @@ def main() -> int:
     ap.add_argument("--tau-fs", type=float, default=0.5, help="Berendsen time constant (0: NVE)")
     ap.add_argument("--judge-every", type=int, default=20)
     ap.add_argument("--fallback-steps", type=int, default=200)
+    ap.add_argument("--probation-every", type=int, default=5,
+                    help="judge interval of a student that failed, until it passes again")
     ap.add_argument("--fallback-precision", default="mixed", choices=("mixed", "float64"),
                     help="precision of the fallback teacher (below the judge's it is judged too)")
     ap.add_argument("--threshold-mev", type=float, default=5.0)
@@ -385,8 +390,8 @@ def main() -> int:
     fb = ("mixed-precision teacher steps (judged the same way; failing -> FP64 teacher)"
           if args.fallback_precision == "mixed" else "FP64 teacher steps")
     head = ("surrogate: student 34 ACSF 34-16-16-1 mixed drives, FP64 teacher judges every "
-            f"{args.judge_every} steps on {judged}, > {args.threshold_mev} meV/Bohr -> rewind + "
-            f"{args.fallback_steps} {fb}") if sur is not None else "mixed-precision teacher (no trained student found)"
+            f"{args.judge_every} steps (a student that failed: every {args.probation_every}) on {judged}, "
+            f"> {args.threshold_mev} meV/Bohr -> rewind + {args.fallback_steps} {fb}") if sur is not None else "mixed-precision teacher (no trained student found)"
     out = {
         "metric": "atom-steps/s, 98k-atom water NNP-MD: mixed+surrogate vs FP64; force RMSE",
         "value": value,

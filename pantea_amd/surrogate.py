@@ -6,7 +6,10 @@ forces are compared with the student's (RMSE over all force components, or
 over the atoms of a random sample when ``judge_atoms`` is set).  When the
 RMSE exceeds ``threshold`` the run rewinds to the last state that passed and
 the teacher drives the next ``fallback_steps`` steps; then the student takes
-over again from the state the teacher reached.
+over again from the state the teacher reached.  A student that has failed
+is on probation until it passes again: it is judged every
+``probation_every`` steps (default ``judge_every``), so a student that still
+fails wastes fewer discarded steps.
 
 The fallback teacher may run at a lower precision than the judge (e.g. the
 mixed-precision teacher under an FP64 judge).  It is then judged like the
@@ -87,13 +90,16 @@ class SurrogateMD:
                  fallback_precision: str = "float64", judge_every: int = 20,
                  threshold: float = 5.0 / units.TO_MILLI_ELECTRON_VOLT, fallback_steps: int = 200,
                  skin: float = 1.0, thermostat: Optional[BrendsenThermostat] = None,
-                 judge_atoms: Optional[int] = None, judge_seed: int = 0):
+                 judge_atoms: Optional[int] = None, judge_seed: int = 0,
+                 probation_every: Optional[int] = None):
         import torch
 
         if integrator not in ("reference", "physical"):
             raise ValueError(f"unknown integrator '{integrator}'")
         if judge_every < 1 or fallback_steps < 1:
             raise ValueError("judge_every and fallback_steps must be >= 1")
+        if probation_every is not None and not 1 <= probation_every <= judge_every:
+            raise ValueError("probation_every must be in [1, judge_every]")
         self.system = System.from_structure(structure, student, temperature=temperature, seed=seed,
                                             force_mode="full", precision=student_precision, skin=skin)
         md = self.system._engine()
@@ -112,6 +118,8 @@ class SurrogateMD:
         self.integrator = integrator
         self.thermostat = thermostat
         self.judge_every = int(judge_every)
+        self.probation_every = int(probation_every) if probation_every is not None else self.judge_every
+        self.on_probation = False
         self.threshold = float(threshold)
         self.fallback_len = int(fallback_steps)
         self.f_judge_out = torch.zeros_like(md.force)
@@ -236,8 +244,12 @@ class SurrogateMD:
             st.steps += 1
             if self.mode == "student":
                 st.student_steps += 1
-                if st.steps - self.good_step >= self.judge_every and not self._judge_step():
-                    self._to_fallback()
+                every = self.probation_every if self.on_probation else self.judge_every
+                if st.steps - self.good_step >= every:
+                    passed = self._judge_step()
+                    self.on_probation = not passed
+                    if not passed:
+                        self._to_fallback()
                 continue
             st.fallback_steps += 1
             if self.mode == "reference":

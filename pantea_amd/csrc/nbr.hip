@@ -122,10 +122,9 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(int n, const int* k
     ghist[threadIdx.x * nblocks + blockIdx.x] = hist[threadIdx.x];
 }
 
-// Exclusive scan of m ints in one block (m up to a few 1e5).
-// Exclusive scan of m ints in place by one block: each thread scans a
-// contiguous chunk of ceil(m / 1024), the chunk totals are scanned across the
-// block (wave shuffles, then the 16 wave totals).
+// Exclusive scan of m ints (up to a few 1e5) in place by one block: each
+// thread scans a contiguous chunk of ceil(m / 1024), the chunk totals are
+// scanned across the block (wave shuffles, then the 16 wave totals).
 __global__ __launch_bounds__(1024) void k_scan_small(int m, int* data) {
     __shared__ int wtot[1024 / kWave];
     const int lane = threadIdx.x & (kWave - 1);
@@ -501,6 +500,14 @@ __global__ __launch_bounds__(256) void k_nbr_cell(const double* __restrict__ pos
             count += __popcll(bal);
         }
         if (!FILL && lane == 0) counts[i] = count;
+        if (PASS == PASS_STAGE && count <= kStageW) {
+            // INT_MAX past the row end, to the end of its last bucket, and in
+            // the heads of unused buckets: the reverse search then needs no
+            // row length (INT_MAX exceeds every atom index)
+            const int nb = (count + kBucket - 1) / kBucket;
+            if (count + lane < nb * kBucket) out_idx[(size_t)i * kStageW + count + lane] = INT_MAX;
+            if (lane >= nb && lane < kNBucket) heads[(size_t)i * kNBucket + lane] = INT_MAX;
+        }
     }
 }
 
@@ -615,10 +622,11 @@ __global__ __launch_bounds__(256) void k_rev(int n, const long long* __restrict_
 // Staged rows -> CSR, and the reverse index of every edge.  Wave i copies
 // its row; for each j > i (a suffix of the ascending row) it finds i in the
 // staged row of j and writes both rev entries, so each unordered pair is
-// searched once.  The search is two 64-byte reads: the bucket heads of row j
-// (first entry of every kBucket entries) select the bucket, then the bucket
-// itself.  The relation is symmetric bit for bit (x_i - x_j and the
-// single-shift image are exactly antisymmetric), so every entry is written.
+// searched once.  The search is two 64-byte reads, each shared by the four
+// lanes of a quad: the bucket heads of row j (first entry of every kBucket
+// entries) select the bucket, then the bucket itself.  The relation is
+// symmetric bit for bit (x_i - x_j and the single-shift image are exactly
+// antisymmetric), so every entry is written.
 __device__ __forceinline__ int count_lt(const int4& v, int base, int lim, int key) {
     return (base + 0 < lim && v.x < key) + (base + 1 < lim && v.y < key) + (base + 2 < lim && v.z < key) +
            (base + 3 < lim && v.w < key);
@@ -651,40 +659,77 @@ __global__ __launch_bounds__(256) void k_compact_rev(int n, const long long* __r
     const long long oi = row_off[i];
     const int ci = counts[i];
     const int* si = stage + (size_t)i * kStageW;
+    // the whole staged row in registers, slot s holding entry s * 64 + lane:
+    // its loads are issued together rather than one round trip per slot
+    constexpr int kSlots = kStageW / kWave;
+    int js[kSlots];
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+        const int t = s * kWave + lane;
+        js[s] = (s * kWave < ci && t < ci) ? si[t] : 0;
+    }
+    // count first, store after: the count waits once for every load, and the
+    // stores then issue back to back (a store placed between them would be
+    // waited for by the next slot's wait)
     int below = 0;   // entries j < i
-    for (int base = 0; base < ci; base += kWave) {
-        const int t = base + lane;
-        const bool in = t < ci;
-        const int j = in ? si[t] : 0;
-        if (in) nbr[oi + t] = j;
-        below += __popcll(__ballot(in && j < i));
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+        const int t = s * kWave + lane;
+        below += __popcll(__ballot(t < ci && js[s] < i));
+    }
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+        const int t = s * kWave + lane;
+        if (t < ci) nbr[oi + t] = js[s];
     }
     if (rev == nullptr) return;
-    for (int t = below + lane; t < ci; t += kWave) {
-        const int j = si[t];   // > i
-        const int cj = counts[j];
-        const int nb = (cj + kBucket - 1) / kBucket;
-        const int4* hp = reinterpret_cast<const int4*>(heads + (size_t)j * kNBucket);
-        const int4 h0 = hp[0], h1 = hp[1], h2 = hp[2], h3 = hp[3];
-        const int b = count_le(h0, 0, nb, i) + count_le(h1, 4, nb, i) + count_le(h2, 8, nb, i) +
-                      count_le(h3, 12, nb, i) - 1;   // bucket whose head is the last <= i
-        bool found = false;
-        int p = 0;
-        if (b >= 0) {
-            const int o = b * kBucket;
-            const int4* bp = reinterpret_cast<const int4*>(stage + (size_t)j * kStageW + o);
-            const int4 v0 = bp[0], v1 = bp[1], v2 = bp[2], v3 = bp[3];
-            p = o + count_lt(v0, o, cj, i) + count_lt(v1, o + 4, cj, i) + count_lt(v2, o + 8, cj, i) +
-                count_lt(v3, o + 12, cj, i);
-            found = has_eq(v0, o, cj, i) || has_eq(v1, o + 4, cj, i) || has_eq(v2, o + 8, cj, i) ||
-                    has_eq(v3, o + 12, cj, i);
+    // the entries j > i (positions below .. ci-1), sixteen per step: the four
+    // lanes of a quad share an entry and each reads one int4 quarter of the
+    // 64-byte heads and bucket, so one load instruction touches 16 rows of
+    // the vector cache instead of 64.  Staged rows carry INT_MAX past their
+    // end and in unused heads, so every compare is against the full 16 heads
+    // and the full 16 entries of the bucket.
+    const int q = lane & 3;
+    const int na = ci - below;
+    for (int base = 0; base < na; base += kWave / 4) {
+        const int e = base + (lane >> 2);
+        const bool act = e < na;
+        const int t = below + e;
+        int j = 0;   // staged entry t, from the slot registers
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            const int v = __shfl(js[s], t & (kWave - 1));
+            if ((t >> 6) == s) j = v;
         }
-        if (found) {
-            const long long oj = row_off[j];
-            rev[oi + t] = (int)(oj + p);
-            rev[oj + p] = (int)(oi + t);
-        } else {
-            rev[oi + t] = -1;   // only if the list were asymmetric
+        long long oj = 0;
+        int4 h = make_int4(INT_MAX, INT_MAX, INT_MAX, INT_MAX);
+        if (act) {
+            oj = row_off[j];
+            h = reinterpret_cast<const int4*>(heads + (size_t)j * kNBucket)[q];
+        }
+        int c = count_le(h, 0, kNBucket, i);
+        c += __shfl_xor(c, 1);
+        c += __shfl_xor(c, 2);
+        const int b = c - 1;   // bucket whose head is the last <= i
+        int lt = 0, eq = 0;
+        if (act && b >= 0) {
+            const int o = b * kBucket;
+            const int4 v = reinterpret_cast<const int4*>(stage + (size_t)j * kStageW + o)[q];
+            lt = count_lt(v, 0, kStageW, i);
+            eq = has_eq(v, 0, kStageW, i);
+        }
+        lt += __shfl_xor(lt, 1);
+        lt += __shfl_xor(lt, 2);
+        eq |= __shfl_xor(eq, 1);
+        eq |= __shfl_xor(eq, 2);
+        if (act && q == 0) {
+            if (eq) {
+                const int p = b * kBucket + lt;
+                rev[oi + t] = (int)(oj + p);
+                rev[oj + p] = (int)(oi + t);
+            } else {
+                rev[oi + t] = -1;   // only if the list were asymmetric
+            }
         }
     }
 }

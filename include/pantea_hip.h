@@ -306,9 +306,12 @@ int pt_kinetic_energy(int n, const double *vel, const double *mass, double *out,
 int pt_berendsen_rescale(int n, double *vel, const double *ke, double dt, double tau,
                          double t0, double kb, pt_stream_t stream);
 
-/* out[0] = max_i |MIC(pos4_i - ref_i)| (device scalar; out holds 257
- * doubles, out[1..256] are scratch); drives the rebuild of skin neighbour
- * lists (no counterpart in the reference, which has no neighbour list). */
+/* out[0], out[1] = the largest and the second largest |MIC(pos4_i - ref_i)|
+ * over the atoms (device scalars; out holds 514 doubles, out[2..513] are
+ * scratch).  A skin list stays exact while out[0] + out[1] <= skin: no pair
+ * has moved by more than the sum of its two displacements.  Drives the
+ * rebuild of skin neighbour lists (no counterpart in the reference, which
+ * has no neighbour list). */
 int pt_max_displacement(int n, const double *pos4, const double *ref, const double *box,
                         double *out, pt_stream_t stream);
 

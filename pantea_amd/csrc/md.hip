@@ -147,12 +147,36 @@ __global__ void k_berendsen(int n, double* vel, const double* ke, double dt, dou
     vel[3 * i] *= lam; vel[3 * i + 1] *= lam; vel[3 * i + 2] *= lam;
 }
 
-// Squared displacement maxima: one partial per block, then one block over
-// the partials; out[0] = max_i |MIC(pos4_i - ref_i)|.
+// The two largest squared displacements (of two different atoms): one pair
+// per block, then one block over the pairs; out[0], out[1] = the largest and
+// the second largest |MIC(pos4_i - ref_i)|.
+__device__ __forceinline__ void top2_merge(double& a1, double& a2, double b1, double b2) {
+    const double lo = fmin(a1, b1);
+    a1 = fmax(a1, b1);
+    a2 = fmax(lo, fmax(a2, b2));
+}
+
+__device__ void top2_block(double& m1, double& m2) {
+    __shared__ double r1[256], r2[256];
+    r1[threadIdx.x] = m1;
+    r2[threadIdx.x] = m2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            double a1 = r1[threadIdx.x], a2 = r2[threadIdx.x];
+            top2_merge(a1, a2, r1[threadIdx.x + s], r2[threadIdx.x + s]);
+            r1[threadIdx.x] = a1;
+            r2[threadIdx.x] = a2;
+        }
+        __syncthreads();
+    }
+    m1 = r1[0];
+    m2 = r2[0];
+}
+
 __global__ __launch_bounds__(256) void k_max_disp_partial(int n, const double* pos4, const double* ref,
                                                           BoxArg box, double* partial) {
-    __shared__ double red[256];
-    double m = 0.0;
+    double m1 = 0.0, m2 = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         double dx = pos4[4 * i] - ref[4 * i], dy = pos4[4 * i + 1] - ref[4 * i + 1],
                dz = pos4[4 * i + 2] - ref[4 * i + 2];
@@ -161,28 +185,17 @@ __global__ __launch_bounds__(256) void k_max_disp_partial(int n, const double* p
             dy = mic1(dy, box.L[1], box.half[1]);
             dz = mic1(dz, box.L[2], box.half[2]);
         }
-        m = fmax(m, dx * dx + dy * dy + dz * dz);
+        top2_merge(m1, m2, dx * dx + dy * dy + dz * dz, 0.0);
     }
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+    top2_block(m1, m2);
+    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = m1; partial[2 * blockIdx.x + 1] = m2; }
 }
 
 __global__ __launch_bounds__(256) void k_max_disp_final(int nb, const double* partial, double* out) {
-    __shared__ double red[256];
-    double m = 0.0;
-    for (int k = threadIdx.x; k < nb; k += 256) m = fmax(m, partial[k]);
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[0] = sqrt(red[0]);
+    double m1 = 0.0, m2 = 0.0;
+    for (int k = threadIdx.x; k < nb; k += 256) top2_merge(m1, m2, partial[2 * k], partial[2 * k + 1]);
+    top2_block(m1, m2);
+    if (threadIdx.x == 0) { out[0] = sqrt(m1); out[1] = sqrt(m2); }
 }
 
 template <typename T>
@@ -315,12 +328,12 @@ extern "C" int pt_max_displacement(int n, const double* pos4, const double* ref,
     BoxArg b{};
     b.periodic = box != nullptr;
     for (int d = 0; d < 3; ++d) { b.L[d] = box ? box[d] : 0; b.half[d] = 0.5 * b.L[d]; }
-    // out[1 .. 1 + kDispBlocks) holds the block partials (out must hold 1 + 256 doubles)
+    // out[2 .. 2 + 2 nb) holds the block pairs (out must hold 2 + 2 * 256 doubles)
     const int nb = std::max(1, std::min(256, (n + 255) / 256));
     hipLaunchKernelGGL(k_max_disp_partial, dim3(nb), dim3(256), 0, (hipStream_t)stream, n, pos4, ref, b,
-                       out + 1);
+                       out + 2);
     hipLaunchKernelGGL(k_max_disp_final, dim3(1), dim3(256), 0, (hipStream_t)stream, nb,
-                       (const double*)(out + 1), out);
+                       (const double*)(out + 2), out);
     return launch_check("pt_max_displacement");
 }
 

@@ -16,9 +16,10 @@ never divides by the mass, molecular_dynamics.py:16-30) and ``"physical"``
 (a = F / m).  Two force modes: ``"reference"`` (the reference's self term)
 and ``"full"`` (-grad E).
 
-Neighbour lists are built with cutoff rc + skin and rebuilt as soon as any
-atom has moved more than skin / 2 since the last build (checked on the
-device every step); the descriptor kernels re-apply the exact mask
+Neighbour lists are built with cutoff rc + skin and rebuilt as soon as the
+two largest atom displacements since the last build sum to more than the
+skin (checked on the device every step: no pair can have moved by more);
+the descriptor kernels re-apply the exact mask
 (r <= rc, r > 0), so every step sees exactly the reference's neighbour set.
 """
 
@@ -166,7 +167,7 @@ class _DeviceMD:
         self.ref_pos = nl.pos4.clone()
         self.force = torch.zeros((n, 3), dtype=torch.float64, device=self.dev)
         self.force_new = torch.zeros_like(self.force)
-        self.disp = torch.zeros(257, dtype=torch.float64, device=self.dev)
+        self.disp = torch.zeros(514, dtype=torch.float64, device=self.dev)
         self.reducer = Reducer(n, self.dev)
         self.n_builds = 1
         self.timers = None  # {phase: [(start_event, end_event), ...]} when profiling
@@ -248,7 +249,8 @@ class _DeviceMD:
         _lib.check(lib.pt_max_displacement(self.n, self.nl.pos4.data_ptr(), self.ref_pos.data_ptr(),
                                            ctypes.cast(box, ctypes.c_void_p) if box is not None else None,
                                            self.disp.data_ptr(), _lib.stream_ptr()), "pt_max_displacement")
-        return float(self.disp[0].item()) > 0.5 * self.skin
+        d1, d2 = self.disp[:2].tolist()   # the two largest displacements
+        return d1 + d2 > self.skin
 
     # ---- state snapshots (rewind) -------------------------------------------------
     def save_state(self) -> dict:

@@ -73,6 +73,41 @@ def test_verlet_step_matches_oracle(integrator, force_mode, skin):
     assert md.step == 3 and md.elapsed_time == pytest.approx(3 * dt)
 
 
+@pytest.mark.parametrize("n", [1, 2, 300, 100003])
+def test_max_displacement_top_two(n):
+    """pt_max_displacement: the largest and second largest minimum-image displacement
+    (two different atoms), which bound every pair's relative motion for the skin-list
+    rebuild test; displacements up to 0.6 box lengths so the image shift acts."""
+    import ctypes
+
+    import torch
+
+    from pantea_amd import _lib
+    from pantea_amd.atoms import host_box
+
+    rng = np.random.default_rng(n)
+    L = np.array([20.0, 23.0, 27.0])
+    ref = rng.uniform(0.0, 1.0, (n, 3)) * L
+    d = rng.normal(0.0, 0.3, (n, 3))
+    d[rng.integers(0, n)] += 0.6 * L   # one far move, wrapped by the image shift
+    pos = ref + d
+    pos4 = np.zeros((n, 4)); pos4[:, :3] = pos
+    ref4 = np.zeros((n, 4)); ref4[:, :3] = ref
+    out = torch.zeros(514, dtype=torch.float64, device="cuda")
+    box = host_box(L)
+    p4, r4 = torch.tensor(pos4, device="cuda"), torch.tensor(ref4, device="cuda")
+    lib = _lib.load()
+    _lib.check(lib.pt_max_displacement(n, p4.data_ptr(), r4.data_ptr(), ctypes.cast(box, ctypes.c_void_p),
+                                       out.data_ptr(), _lib.stream_ptr()), "pt_max_displacement")
+    got = out[:2].cpu().numpy()
+    dd = pos - ref
+    dd = np.where(dd > 0.5 * L, dd - L, dd)
+    dd = np.where(dd < -0.5 * L, dd + L, dd)
+    r = np.sort(np.sqrt((dd * dd).sum(1)))[::-1]
+    want = np.array([r[0], r[1] if n > 1 else 0.0])
+    assert np.allclose(got, want, rtol=1e-14, atol=0.0), (got, want)
+
+
 def test_berendsen_step_matches_oracle():
     from pantea_amd.simulation import BrendsenThermostat, MDSimulator, System
     from pantea_amd.units import units

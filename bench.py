@@ -9,7 +9,8 @@ structure.  Legs, all on the same structure and protocol:
 
   surrogate (headline): the trained student (34 ACSF, MLP 34-16-16-1, mixed
       precision) drives; the FP64 teacher judges every 20 steps (every 5
-      while the student is on probation after a failure); a force RMSE above
+      while the student is on probation: until its first pass and after a
+      failure); a force RMSE above
       5 meV/Bohr rewinds to the last good state and the mixed-precision
       teacher drives the next 200 steps, judged the same way (a failure there
       hands the window to the FP64 teacher).
@@ -231,7 +232,7 @@ def main() -> int:
     ap.add_argument("--judge-every", type=int, default=20)
     ap.add_argument("--fallback-steps", type=int, default=200)
     ap.add_argument("--probation-every", type=int, default=5,
-                    help="judge interval of a student that failed, until it passes again")
+                    help="judge interval of a student on probation (not yet passed, or failed), until it passes")
     ap.add_argument("--fallback-precision", default="mixed", choices=("mixed", "float64"),
                     help="precision of the fallback teacher (below the judge's it is judged too)")
     ap.add_argument("--threshold-mev", type=float, default=5.0)
@@ -390,7 +391,7 @@ def main() -> int:
     fb = ("mixed-precision teacher steps (judged the same way; failing -> FP64 teacher)"
           if args.fallback_precision == "mixed" else "FP64 teacher steps")
     head = ("surrogate: student 34 ACSF 34-16-16-1 mixed drives, FP64 teacher judges every "
-            f"{args.judge_every} steps (a student that failed: every {args.probation_every}) on {judged}, "
+            f"{args.judge_every} steps (a student on probation, until it passes: every {args.probation_every}) on {judged}, "
             f"> {args.threshold_mev} meV/Bohr -> rewind + {args.fallback_steps} {fb}") if sur is not None else "mixed-precision teacher (no trained student found)"
     out = {
         "metric": "atom-steps/s, 98k-atom water NNP-MD: mixed+surrogate vs FP64; force RMSE",

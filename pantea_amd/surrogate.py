@@ -6,10 +6,10 @@ forces are compared with the student's (RMSE over all force components, or
 over the atoms of a random sample when ``judge_atoms`` is set).  When the
 RMSE exceeds ``threshold`` the run rewinds to the last state that passed and
 the teacher drives the next ``fallback_steps`` steps; then the student takes
-over again from the state the teacher reached.  A student that has failed
-is on probation until it passes again: it is judged every
-``probation_every`` steps (default ``judge_every``), so a student that still
-fails wastes fewer discarded steps.
+over again from the state the teacher reached.  A student is on probation
+until it has passed a judge, and again after every failure: it is then
+judged every ``probation_every`` steps (default ``judge_every``), so a
+student that fails wastes fewer discarded steps.
 
 The fallback teacher may run at a lower precision than the judge (e.g. the
 mixed-precision teacher under an FP64 judge).  It is then judged like the
@@ -119,7 +119,7 @@ class SurrogateMD:
         self.thermostat = thermostat
         self.judge_every = int(judge_every)
         self.probation_every = int(probation_every) if probation_every is not None else self.judge_every
-        self.on_probation = False
+        self.on_probation = True   # until the student passes its first judge
         self.threshold = float(threshold)
         self.fallback_len = int(fallback_steps)
         self.f_judge_out = torch.zeros_like(md.force)

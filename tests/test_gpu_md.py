@@ -256,8 +256,9 @@ def test_surrogate_failed_judge_rewinds_to_teacher_trajectory():
 
 
 def test_surrogate_probation_judges_a_failed_student_sooner():
-    """After a failure the student is judged every probation_every steps until it passes:
-    threshold 0 rejects every window, after 4 student steps at first and 2 afterwards."""
+    """A student is judged every probation_every steps until it passes a judge (from the
+    start, and again after each failure): threshold 0 rejects every window after 2 student
+    steps, where judge_every would take 4."""
     from pantea_amd.bench_models import make_nnp, student_acsf
     from pantea_amd.simulation import MDSimulator, System
     from pantea_amd.surrogate import SurrogateMD
@@ -269,10 +270,10 @@ def test_surrogate_probation_judges_a_failed_student_sooner():
     sur = SurrogateMD(teacher, student, s, seed=8, judge_every=4, threshold=0.0, fallback_steps=10, skin=1.0,
                       probation_every=2)
     st = sur.run(25)
-    # 4 student steps rejected, 10 teacher, 2 rejected, 10 teacher, 2 rejected, 5 teacher
+    # 2 student steps rejected, 10 teacher, 2 rejected, 10 teacher, 2 rejected, 5 teacher
     assert st.steps == 25 and st.fallback_steps == 25
     assert st.failures == st.judges == 3
-    assert st.discarded_steps == 4 + 2 + 2
+    assert st.discarded_steps == 2 + 2 + 2
     plain = System.from_structure(s, teacher, temperature=300.0, seed=8, force_mode="full", precision="float64",
                                   skin=1.0)
     MDSimulator(0.5 * units.FROM_FEMTO_SECOND, integrator="physical").run(plain, 25)

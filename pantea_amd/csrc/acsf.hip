@@ -258,7 +258,8 @@ struct Stage {
     V2<R>* Q2;     // FWD: (r, fc)
     V2<R>* QF;     // ADJ: (fc, dfc)
     R* D;          // [3][cap] TAN: delta_i - delta_j
-    double* DA;    // [3][cap] ADJ: per-edge accumulators (FP64: fast LDS atomics)
+    double* DA;    // [cap][3] ADJ: per-edge accumulators (FP64: fast LDS atomics; x, y, z
+                   // adjacent so the three adds of an edge share one address)
     unsigned long long* keep;   // ADJ: the staging ballots, one per 64 row entries
     unsigned* queue;
     R* W;          // ADJ, FP64: the group's dE/dG weights (read by broadcast)
@@ -422,6 +423,25 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
     for (int e = 0; e < PT_AG_ETA; ++e)
 #pragma unroll
         for (int q = 0; q < PT_AG_ANG; ++q) W[e][q] = kPreOut ? Win[e][q] * X.pre[q] : Win[e][q];
+    // Adjoint of the zeta {1, 4} x lambda +-1 class: the weighted angular sum
+    // of one eta, U(c) = w0 (1 + c) + w1 (1 - c) + w2 (1 + c)^4 + w3 (1 - c)^4,
+    // is a quartic in c.  With sp = w2 + w3, sm = w2 - w3, W[e] becomes
+    // (u0, u1, sp, sm4) = (w0 + w1 + sp, w0 - w1 + 4 sm, sp, 4 sm) and
+    //   U = u0 + u1 c + sp c^2 (6 + c^2) + sm4 c^3,
+    //   U' = u1 + sp (12 c + 4 c^3) + 3 sm4 c^2,
+    // so the pair body needs no angular terms, and the powers of c are
+    // shared by the eta values.
+    constexpr bool kQuartic = MODE == MODE_ADJ && ANG == ANGK_Z14;
+    if (kQuartic) {
+#pragma unroll
+        for (int e = 0; e < PT_AG_ETA; ++e) {
+            const R w0 = W[e][0], w1 = W[e][1], sp = W[e][2] + W[e][3], sm = W[e][2] - W[e][3];
+            W[e][0] = (w0 + w1) + sp;
+            W[e][1] = (w0 - w1) + R(4) * sm;
+            W[e][2] = sp;
+            W[e][3] = R(4) * sm;
+        }
+    }
     if (kWLds) {
         if (lane == 0) {
 #pragma unroll
@@ -487,7 +507,7 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
         const R iab = ira * irb;
         const R cs = (pa.x * pb.x + pa.y * pb.y + pa.z * pb.z) * iab;
         R ang[PT_AG_ANG], dang[PT_AG_ANG];
-        ang_eval<R, ANG, MODE != MODE_FWD>(X, cs, ang, dang);
+        if constexpr (!kQuartic) ang_eval<R, ANG, MODE != MODE_FWD>(X, cs, ang, dang);
         if (MODE == MODE_FWD) {
 #pragma unroll
             for (int e = 0; e < PT_AG_ETA; ++e) {
@@ -507,16 +527,33 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
         const R Fa = qa.w * qb.z * fjk, Fb = qa.z * qb.w * fjk, Fj = qa.z * qb.z * dfjk;
         if (MODE == MODE_ADJ) {
             R SP = R(0), SKP = R(0), SV = R(0);
+            R c3 = R(0), t4 = R(0), dt4 = R(0), c2x3 = R(0);
+            if constexpr (kQuartic) {
+                const R c2 = cs * cs;
+                c3 = c2 * cs;
+                t4 = c2 * (R(6) + c2);
+                dt4 = R(12) * cs + R(4) * c3;
+                c2x3 = R(3) * c2;
+            }
 #pragma unroll
             for (int e = 0; e < PT_AG_ETA; ++e) {
                 if (e < X.n_eta) {
                     const R ex = M<R>::ex2k(X.keta[e] * Ssum);
                     R U = R(0), V = R(0);
+                    if constexpr (kQuartic) {
+                        const R u0 = kWLds ? S.W[e * PT_AG_ANG + 0] : W[e][0];
+                        const R u1 = kWLds ? S.W[e * PT_AG_ANG + 1] : W[e][1];
+                        const R sp = kWLds ? S.W[e * PT_AG_ANG + 2] : W[e][2];
+                        const R sm4 = kWLds ? S.W[e * PT_AG_ANG + 3] : W[e][3];
+                        U = u0 + u1 * cs + sp * t4 + sm4 * c3;
+                        V = u1 + sp * dt4 + sm4 * c2x3;
+                    } else {
 #pragma unroll
-                    for (int q = 0; q < NA; ++q) {
-                        const R wq = kWLds ? S.W[e * PT_AG_ANG + q] : W[e][q];
-                        U += wq * ang[q];
-                        V += wq * dang[q];
+                        for (int q = 0; q < NA; ++q) {
+                            const R wq = kWLds ? S.W[e * PT_AG_ANG + q] : W[e][q];
+                            U += wq * ang[q];
+                            V += wq * dang[q];
+                        }
                     }
                     const R P = ex * U;
                     SP += P;
@@ -543,10 +580,10 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
             const R gbx = bl * pb.x + be * pa.x - sj * jx;
             const R gby = bl * pb.y + be * pa.y - sj * jy;
             const R gbz = bl * pb.z + be * pa.z - sj * jz;
-            lds_add(&S.DA[ia], -(double)gax); lds_add(&S.DA[cap + ia], -(double)gay);
-            lds_add(&S.DA[2 * cap + ia], -(double)gaz);
-            lds_add(&S.DA[ib], -(double)gbx); lds_add(&S.DA[cap + ib], -(double)gby);
-            lds_add(&S.DA[2 * cap + ib], -(double)gbz);
+            lds_add(&S.DA[3 * ia], -(double)gax); lds_add(&S.DA[3 * ia + 1], -(double)gay);
+            lds_add(&S.DA[3 * ia + 2], -(double)gaz);
+            lds_add(&S.DA[3 * ib], -(double)gbx); lds_add(&S.DA[3 * ib + 1], -(double)gby);
+            lds_add(&S.DA[3 * ib + 2], -(double)gbz);
         } else {  // MODE_TAN
             const R cax = pb.x * iab - cs * pa.x * ira2, cay = pb.y * iab - cs * pa.y * ira2,
                     caz = pb.z * iab - cs * pa.z * ira2;  // d cos / d R_ij
@@ -751,7 +788,7 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
             } else {
                 S.Q[s] = qv;
             }
-            if (MODE == MODE_ADJ) { S.DA[s] = 0.0; S.DA[cap + s] = 0.0; S.DA[2 * cap + s] = 0.0; }
+            if (MODE == MODE_ADJ) { S.DA[3 * s] = 0.0; S.DA[3 * s + 1] = 0.0; S.DA[3 * s + 2] = 0.0; }
             if (MODE == MODE_TAN) {
                 double ex = dix, ey = diy, ez = diz;
                 if (!a.center_only) {
@@ -820,7 +857,7 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
             if (MODE == MODE_ADJ) {
                 const R sc = dsum * ((MODE == MODE_ADJ) ? M<R>::rcp(pv.w) : pv.w);
                 const R gx = sc * pv.x, gy = sc * pv.y, gz = sc * pv.z;
-                S.DA[s] -= (double)gx; S.DA[cap + s] -= (double)gy; S.DA[2 * cap + s] -= (double)gz;
+                S.DA[3 * s] -= (double)gx; S.DA[3 * s + 1] -= (double)gy; S.DA[3 * s + 2] -= (double)gz;
             }
         }
         if (MODE != MODE_ADJ) {
@@ -958,9 +995,9 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
                 if ((bal >> lane) & 1ull) {
                     const int s = ms + lanes_below(bal);
                     const long long e = off + t0 + lane;
-                    eg[3 * e + 0] = bad ? nan : (R)S.DA[s];
-                    eg[3 * e + 1] = bad ? nan : (R)S.DA[cap + s];
-                    eg[3 * e + 2] = bad ? nan : (R)S.DA[2 * cap + s];
+                    eg[3 * e + 0] = bad ? nan : (R)S.DA[3 * s];
+                    eg[3 * e + 1] = bad ? nan : (R)S.DA[3 * s + 1];
+                    eg[3 * e + 2] = bad ? nan : (R)S.DA[3 * s + 2];
                 }
                 ms += __popcll(bal);
             }
@@ -970,9 +1007,9 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
             // and F_self = -dE_i/dr_i = sum_j dE_i/dr_j, the per-edge accumulators
             double fx = 0.0, fy = 0.0, fz = 0.0;
             for (int s = lane; s < m; s += kWave) {
-                fx += S.DA[s];
-                fy += S.DA[cap + s];
-                fz += S.DA[2 * cap + s];
+                fx += S.DA[3 * s];
+                fy += S.DA[3 * s + 1];
+                fz += S.DA[3 * s + 2];
             }
             const double sx = wave_sum(fx), sy = wave_sum(fy), sz = wave_sum(fz);
             if (lane == 0) {

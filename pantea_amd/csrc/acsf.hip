@@ -253,10 +253,10 @@ __host__ __device__ constexpr int queue_len() { return 128; }
 
 template <typename R>
 struct Stage {
-    V4<R>* P;      // (x, y, z, 1/r) of R_ij = r_i - r_j; ADJ: (x, y, z, r)
+    V4<R>* P;      // (x, y, z, 1/r) of R_ij = r_i - r_j
     V4<R>* Q;      // TAN: (r, r^2, fc, dfc/dr) -- cutoff of the current group
     V2<R>* Q2;     // FWD: (r, fc)
-    V2<R>* QF;     // ADJ: (fc, dfc)
+    V2<R>* QF;     // ADJ: (fc, dfc / r) of the current group
     R* D;          // [3][cap] TAN: delta_i - delta_j
     double* DA;    // [cap][3] ADJ: per-edge accumulators (FP64: fast LDS atomics; x, y, z
                    // adjacent so the three adds of an edge share one address)
@@ -265,9 +265,9 @@ struct Stage {
     R* W;          // ADJ, FP64: the group's dE/dG weights (read by broadcast)
 };
 
-// The adjoint keeps a compact record (x, y, z, r) + (fc, dfc) and no
-// per-neighbour row slot (1/r is recomputed per pair, edge positions come
-// back from the staging ballots): the LDS per wave then allows three waves
+// The adjoint keeps a compact record (x, y, z, 1/r) + per group (fc, dfc / r)
+// and no per-neighbour row slot (r^2 is recomputed per pair from x, y, z,
+// edge positions come back from the staging ballots): the LDS per wave then allows three waves
 // per SIMD in FP64 and four in FP32.
 
 template <typename R, int MODE>
@@ -456,9 +456,10 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
         for (int s = sb + lane; s < sb + sn; s += kWave) {
             R f, df;
             if constexpr ((MODE == MODE_ADJ)) {
-                cutoff_k<R, CUT>(X.cut, X.rc, X.irc, X.cpre, X.tanh_pre, S.P[s].w, f, df);
+                const R ir = S.P[s].w;
+                cutoff_k<R, CUT>(X.cut, X.rc, X.irc, X.cpre, X.tanh_pre, M<R>::rcp(ir), f, df);
                 V2<R> fv;
-                fv.x = f; fv.y = df;
+                fv.x = f; fv.y = df * ir;   // dfc / r
                 S.QF[s] = fv;
             } else if constexpr (MODE == MODE_FWD) {
                 cutoff_k<R, CUT>(X.cut, X.rc, X.irc, X.cpre, X.tanh_pre, S.Q2[s].x, f, df);
@@ -478,11 +479,14 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
         V4<R> qa, qb;   // (r, r^2, fc, dfc)
         R ira, irb;     // 1 / r
         if constexpr ((MODE == MODE_ADJ)) {
+            // qa.w = dfc / r here
             const V2<R> ua = S.QF[ia], ub = S.QF[ib];
-            qa.x = pa.w; qa.y = pa.w * pa.w; qa.z = ua.x; qa.w = ua.y;
-            qb.x = pb.w; qb.y = pb.w * pb.w; qb.z = ub.x; qb.w = ub.y;
-            ira = M<R>::rcp(pa.w);
-            irb = M<R>::rcp(pb.w);
+            ira = pa.w;
+            irb = pb.w;
+            qa.y = pa.x * pa.x + pa.y * pa.y + pa.z * pa.z;
+            qb.y = pb.x * pb.x + pb.y * pb.y + pb.z * pb.z;
+            qa.x = R(0); qa.z = ua.x; qa.w = ua.y;
+            qb.x = R(0); qb.z = ub.x; qb.w = ub.y;
         } else if constexpr (MODE == MODE_FWD) {
             const V2<R> ua = S.Q2[ia], ub = S.Q2[ib];
             qa.x = ua.x; qa.y = ua.x * ua.x; qa.z = ua.y; qa.w = R(0);
@@ -562,17 +566,17 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
                 }
             }
             // dT/dr = d/dr [exp(-eta S) F]: F' sum_e P_e - 2 r F sum_e eta_e P_e,
-            // sum_e eta_e P_e = -SKP / kexp
-            const R F2 = F * R(2.0 / M<R>::kexp);
-            const R d_rij = Fa * SP + ra * F2 * SKP;
-            const R d_rik = Fb * SP + rb * F2 * SKP;
-            const R d_rjk = X.g3 ? (Fj * SP + rjk * F2 * SKP) : R(0);
+            // sum_e eta_e P_e = -SKP / kexp.  With Fa, Fb built from dfc / r,
+            // d T / d r_ij / r_ij = Fa SP + F2 SKP (and the same for r_ik).
+            const R F2S = F * R(2.0 / M<R>::kexp) * SKP;
+            const R d_rjk = X.g3 ? (Fj * SP + rjk * F2S) : R(0);
             const R d_c = F * SV;
             // d cos / d R_ij = R_ik/(r_ij r_ik) - cos R_ij / r_ij^2 (and a <-> b), folded:
-            // g_a = (s_a - d_c cos / r_ij^2) R_ij + (d_c / (r_ij r_ik)) R_ik + s_j R_jk
+            // g_a = (d_rij / r_ij - d_c cos / r_ij^2) R_ij + (d_c / (r_ij r_ik)) R_ik + s_j R_jk
             const R sj = d_rjk * irjk;
-            const R al = d_rij * ira - d_c * cs * ira2;
-            const R bl = d_rik * irb - d_c * cs * irb2;
+            const R dcc = d_c * cs;
+            const R al = Fa * SP + F2S - dcc * ira2;
+            const R bl = Fb * SP + F2S - dcc * irb2;
             const R be = d_c * iab;
             const R gax = al * pa.x + be * pb.x + sj * jx;
             const R gay = al * pa.y + be * pb.y + sj * jy;
@@ -776,7 +780,7 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
             const int s = m + lanes_below(bal);
             V4<R> pv, qv;
             pv.x = (R)dx; pv.y = (R)dy; pv.z = (R)dz;
-            pv.w = (MODE == MODE_ADJ) ? (R)r : (R)(1.0 / r);
+            pv.w = (R)(1.0 / r);
             qv.x = (R)r; qv.y = (R)r * (R)r; qv.z = R(0); qv.w = R(0);
             S.P[s] = pv;
             if constexpr ((MODE == MODE_ADJ)) {
@@ -833,7 +837,7 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
         }
         for (int s = b + lane; s < e; s += kWave) {
             R r;
-            if constexpr ((MODE == MODE_ADJ)) r = S.P[s].w;
+            if constexpr ((MODE == MODE_ADJ)) r = M<R>::rcp(S.P[s].w);
             else if constexpr (MODE == MODE_FWD) r = S.Q2[s].x;
             else r = S.Q[s].x;
             R f, df;
@@ -855,7 +859,7 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
                 }
             }
             if (MODE == MODE_ADJ) {
-                const R sc = dsum * ((MODE == MODE_ADJ) ? M<R>::rcp(pv.w) : pv.w);
+                const R sc = dsum * pv.w;
                 const R gx = sc * pv.x, gy = sc * pv.y, gz = sc * pv.z;
                 S.DA[3 * s] -= (double)gx; S.DA[3 * s + 1] -= (double)gy; S.DA[3 * s + 2] -= (double)gz;
             }

@@ -405,7 +405,7 @@ def test_cfg1_subset_and_fd(dev):
     assert worst < 1e-5
 
 
-@pytest.mark.parametrize("cutoff", ["tanh", "tanhu", "cos", "exp", "poly2"])
+@pytest.mark.parametrize("cutoff", ["tanh", "tanhu", "cos", "exp", "poly1", "poly2", "hard"])
 @pytest.mark.parametrize("zetas,kind", [((1.0, 2.0), "G3"), ((2.0, 3.0), "G3"), ((2.5, 1.0), "G3"), ((1.0, 4.0), "G9"),
                                         ((4.0, 1.0), "G3")])
 def test_nnp_cutoff_and_zeta_paths_vs_oracle(dev, cutoff, zetas, kind):

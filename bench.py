@@ -334,12 +334,15 @@ def main() -> int:
     fwd_ops, adj_ops = algorithmic_ops(n_nb, n_trip, 8, 6 if is_student else 12)
     mode = 1 if ph == "acsf_adjoint" else 0
     total_ms, groups = phases[dom]
-    launch_ms = total_ms / (2.0 * groups)           # each group launches O centres, then H centres
-    launch_ops = (adj_ops if mode else fwd_ops) * n / 2.0
+    pot_of = student if is_student else teacher
+    # elements with byte-identical descriptor tables share one launch per evaluation
+    n_launch = len({bytes(pot_of.atomic_potentials[e].descriptor.device_table(s.element_map, s.device).host)
+                    for e in pot_of.elements})
+    launch_ms = total_ms / (n_launch * groups)
+    launch_ops = (adj_ops if mode else fwd_ops) * n / n_launch
     achieved = launch_ops / (launch_ms * 1e-3) / 1e12 if launch_ms > 0 else 0.0
     peak = PEAK_FP64_TFLOPS if fp64 else PEAK_FP32_TFLOPS
     # the launch's kernel: table class (pt_sf_table_kind) and the four-/eight-wave forward variants
-    pot_of = student if is_student else teacher
     kind = pot_of.atomic_potentials["O"].descriptor.device_table(s.element_map, s.device).kind
     if mode == 0 and fp64 and kind != 0:
         variant = "_w4"
@@ -361,8 +364,10 @@ def main() -> int:
         "note": "vector-ALU bound work (no MFMA in this kernel; 'mfma' is the compute roofline): "
                 f"algorithmic ops per atom {('2.5 x ' if mode else '')}((15 + 6 g2) n_nb + (26 + 5 g3) n_trip), "
                 f"g2 = 8, g3 = {6 if is_student else 12} per neighbour pair type, transcendental = 1, FMA = 2; "
-                f"measured n_nb={n_nb:.1f}, n_trip={n_trip:.0f} per atom; a launch covers the O or the H centres "
-                f"(n/2 on average); average launch time from HIP events over {groups} launch pairs of an "
+                f"measured n_nb={n_nb:.1f}, n_trip={n_trip:.0f} per atom; a launch covers "
+                f"{'all' if n_launch == 1 else f'1/{n_launch} of the'} centres (the O and H tables are "
+                f"{'identical' if n_launch == 1 else 'different'}); average launch time from HIP events over "
+                f"{groups * n_launch} launches of an "
                 f"instrumented {phase_steps}-step repeat of the headline run; peak = published MI355X {'FP64' if fp64 else 'FP32'} vector; traffic = HBM bytes "
                 "per launch from the FETCH_SIZE/WRITE_SIZE profiles, copy-kernel calibrated",
     }

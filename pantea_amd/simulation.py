@@ -72,6 +72,9 @@ class _ForceField:
         self.reducer = Reducer(n, dev)
         self.tables = {e: pot.atomic_potentials[e].descriptor.device_table(emap, dev)
                        for e in pot.elements if e in emap.element_to_atom_type}
+        # descriptor tables hold no centre element: elements whose tables are
+        # byte-identical share one launch (one grid tail instead of several)
+        self._table_key = {e: bytes(t.host) for e, t in self.tables.items()}
         self.covered = all(e in pot.atomic_potentials for e in emap.unique_elements)
         self.mlps = pot._device_mlps(dev)
         self.energy = None
@@ -89,14 +92,26 @@ class _ForceField:
             if cnt > 0:
                 yield e, b, cnt
 
+    def _desc_runs(self, nl: NeighborList):
+        """(table, first centre, centres) of each descriptor launch: adjacent element
+        blocks (internal order is type-sorted) with identical tables form one run."""
+        runs = []
+        for e, b, cnt in sorted(self._blocks(nl), key=lambda x: x[1]):
+            if runs and runs[-1][0] == self._table_key[e] and runs[-1][2] + runs[-1][3] == b:
+                runs[-1][3] += cnt
+            else:
+                runs.append([self._table_key[e], e, b, cnt])
+        return [(self.tables[e], b, cnt) for _, e, b, cnt in runs]
+
     def compute(self, nl: NeighborList, out, force_mode: str, with_energy: bool = False, mark=None):
         import torch
 
         mark = mark or (lambda phase, start: None)
         blocks = list(self._blocks(nl))
+        runs = self._desc_runs(nl)
         mark("acsf_forward", True)
-        for e, b, cnt in blocks:
-            acsf_forward(nl, self.tables[e], self.g_prec, cnt, center_begin=b, out=self.G[b:], ld=self.ld)
+        for tab, b, cnt in runs:
+            acsf_forward(nl, tab, self.g_prec, cnt, center_begin=b, out=self.G[b:], ld=self.ld)
         mark("acsf_forward", False)
         mark("mlp", True)
         for e, b, cnt in blocks:
@@ -117,8 +132,8 @@ class _ForceField:
         if not self.covered:
             target.zero_()
         mark("acsf_adjoint", True)
-        for e, b, cnt in blocks:
-            acsf_adjoint(nl, self.tables[e], self.g_prec, cnt, self.dEdG[b:], self.ld, center_begin=b,
+        for tab, b, cnt in runs:
+            acsf_adjoint(nl, tab, self.g_prec, cnt, self.dEdG[b:], self.ld, center_begin=b,
                          f_self=target[b:], edge_grad=self.edge if full else None)
         mark("acsf_adjoint", False)
         if full:

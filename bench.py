@@ -44,6 +44,7 @@ sys.path.insert(0, str(ROOT))
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
+kWaveThreads = 64   # a descriptor launch runs one wavefront per centre
 
 
 def _log(msg: str) -> None:
@@ -88,8 +89,9 @@ def algorithmic_ops(n_nb, n_trip, g2_per_element=8, g3_per_pair=12):
     return fwd, 2.5 * fwd
 
 
-def profiled_traffic(kernel: str):
-    """HBM bytes per launch of ``kernel`` (mean over its launch shapes) from the committed PMC summary."""
+def profiled_traffic(kernel: str, grid: int):
+    """HBM bytes per launch of ``kernel`` at ``grid`` work-items (mean over those dispatches) from the
+    committed PMC summary."""
     path = ROOT / "profiles" / "hbm_traffic_water98k_md.md"
     if not path.exists():
         return None
@@ -97,7 +99,8 @@ def profiled_traffic(kernel: str):
     for line in path.read_text().splitlines():
         if line.startswith(f"| `{kernel}` |"):
             cols = [c.strip() for c in line.strip("|").split("|")]
-            vals.append((float(cols[5]) + float(cols[6])) * 1e6)
+            if int(cols[1]) == grid:   # the dispatches of this launch size
+                vals.append((float(cols[5]) + float(cols[6])) * 1e6)
     return float(np.mean(vals)) if vals else None
 
 
@@ -357,7 +360,7 @@ def main() -> int:
         "peak": peak,
         "unit": "TFLOP/s",
         "frac": achieved / peak,
-        "traffic": profiled_traffic(kname),
+        "traffic": profiled_traffic(kname, int(kWaveThreads * n / n_launch)),
         "kernel": f"{kname} ({'descriptor adjoint' if mode else 'descriptor forward'}, {label})",
         "avg_launch_ms": launch_ms,
         "algorithmic_ops_per_launch": launch_ops,

@@ -73,6 +73,42 @@ def test_verlet_step_matches_oracle(integrator, force_mode, skin):
     assert md.step == 3 and md.elapsed_time == pytest.approx(3 * dt)
 
 
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-10), ("mixed", 3e-3)])
+def test_md_forces_with_distinct_element_tables(precision, tol):
+    """The MD engine launches the descriptor kernels once for elements whose tables are
+    byte-identical (the teacher's O and H) and once per table otherwise: with an O table
+    that differs from the H table (other radial eta values, other function counts) its
+    -grad E matches the oracle's."""
+    from pantea_amd.bench_models import make_nnp
+    from pantea_amd.descriptors import ACSF, G2, G3, CutoffFunction, NeighborElements
+    from pantea_amd.simulation import System
+
+    s, _ = _cfg0()
+    cfn = CutoffFunction.from_type("tanh", 9.0)
+
+    def acsf_of(el):
+        etas = (0.02, 0.05) if el == "O" else (0.01, 0.03, 0.08)
+        radial = tuple((G2(cfn, 0.0, eta), NeighborElements(e)) for e in ("H", "O") for eta in etas)
+        angular = tuple((G3(cfn, eta=0.005, zeta=z, lambda0=lam, r_shift=0.0), NeighborElements(j, k))
+                        for (j, k) in (("H", "H"), ("H", "O"), ("O", "O")) for z in (1.0, 4.0)
+                        for lam in (1.0, -1.0))
+        return ACSF(el, radial, angular)
+
+    pot = make_nnp(acsf_of, (8, 8), seed=5, precision="float64")
+    pot.fit_scaler([s])
+    _, _, _, f_full = oracle_nnp(pot, s).energy_forces(s.positions.cpu().numpy(), s.atom_types_host,
+                                                       s.box_lengths)
+    sysm = System.from_structure(s, pot, temperature=300.0, seed=3, force_mode="full", precision=precision,
+                                 skin=1.0)
+    eng = sysm._engine()
+    assert len(eng.ff._desc_runs(eng.nl)) == 2
+    sysm.update_forces_from_positions()
+    f = sysm.structure.forces.cpu().numpy()
+    err = float(np.max(np.abs(f - f_full)) / np.max(np.abs(f_full)))
+    print(f"{precision}: max force error {err:.2e} of max |F|")
+    assert err < tol
+
+
 @pytest.mark.parametrize("n", [1, 2, 300, 100003])
 def test_max_displacement_top_two(n):
     """pt_max_displacement: the largest and second largest minimum-image displacement

@@ -41,6 +41,7 @@ constexpr int kScanBlock = 1024;  // items per block of the row-offset scan
 constexpr int kStageW = 256;      // staged entries per row
 constexpr int kBucket = 16;       // entries per bucket of a staged row
 constexpr int kNBucket = kStageW / kBucket;
+constexpr int kCellThreads = 512;  // threads of a candidate-pass workgroup (one per cell)
 
 struct Grid {
     int nc[3];
@@ -316,13 +317,13 @@ constexpr int kCand = 1536;   // candidates of one cell staged in LDS (a multipl
 
 // One workgroup per cell: the candidate ranges of the cell's stencil (the
 // same for all its centres) are built once, the candidates (FP32 position and
-// index) staged in LDS when they fit, and the four wavefronts take the
-// cell's centres in turn, 64 candidates per step.  A cell with more than
+// index) staged in LDS when they fit, and the workgroup's wavefronts take
+// the cell's centres in turn, 64 candidates per step.  A cell with more than
 // kCand candidates reads them from global memory instead (index by binary
 // search over the ranges).  Rows come out in candidate order: type-major,
 // stencil cells ascending, i.e. ascending internal index.
 template <int PASS>
-__global__ __launch_bounds__(256) void k_nbr_cell(const double* __restrict__ pos4,
+__global__ __launch_bounds__(kCellThreads) void k_nbr_cell(const double* __restrict__ pos4,
                                                   const float4* __restrict__ pos4f,
                                                   const unsigned long long* __restrict__ bmax,
                                                   const int* __restrict__ cell_start, Grid g,
@@ -425,7 +426,7 @@ __global__ __launch_bounds__(256) void k_nbr_cell(const double* __restrict__ pos
     const int total = s_rpre[nr];
     const bool staged = total <= kCand;
     if (staged) {
-        for (int t = threadIdx.x; t < total; t += 256) {
+        for (int t = threadIdx.x; t < total; t += kCellThreads) {
             int lo = 0, hi = nr;  // largest r with s_rpre[r] <= t
             while (hi - lo > 1) {
                 const int mid = (lo + hi) >> 1;
@@ -445,7 +446,7 @@ __global__ __launch_bounds__(256) void k_nbr_cell(const double* __restrict__ pos
     const float3 bf = make_float3((float)box.L[0], (float)box.L[1], (float)box.L[2]);
     const float3 ib = make_float3(1.0f / bf.x, 1.0f / bf.y, 1.0f / bf.z);
     const float3 lim = make_float3(1.45f * bf.x, 1.45f * bf.y, 1.45f * bf.z);
-    for (int k = w; k < ncent; k += 4) {
+    for (int k = w; k < ncent; k += kCellThreads / kWave) {
         int t0 = 0;
         while (t0 + 1 < n_types && s_cpre[t0 + 1] <= k) ++t0;
         const int i = uniform(s_cbeg[t0] + (k - s_cpre[t0]));
@@ -910,7 +911,7 @@ extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int*
 
     // count + stage, scan, then compact (or a second pass for rows > kStageW)
     const int nwb = (n + 3) / 4;
-    hipLaunchKernelGGL(k_nbr_cell<PASS_STAGE>, dim3(g.ncell), dim3(256), 0, stream, pos4,
+    hipLaunchKernelGGL(k_nbr_cell<PASS_STAGE>, dim3(g.ncell), dim3(kCellThreads), 0, stream, pos4,
                        (const float4*)pos4f, (const unsigned long long*)bmax, cell_start,
                        g, ba, n_types, thr, counts, (const long long*)nullptr, stage, heads);
     if (int rcode = launch_check("k_nbr_cell<stage>")) return rcode;
@@ -948,7 +949,7 @@ extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int*
     }
     PT_REQUIRE(nbr != nullptr || host_tm[0] == 0, "pt_nbr_build: nbr is null");
     if (host_tm[1] <= kStageW) return PT_OK;
-    hipLaunchKernelGGL(k_nbr_cell<PASS_FILL>, dim3(g.ncell), dim3(256), 0, stream, pos4,
+    hipLaunchKernelGGL(k_nbr_cell<PASS_FILL>, dim3(g.ncell), dim3(kCellThreads), 0, stream, pos4,
                        (const float4*)pos4f, (const unsigned long long*)bmax, cell_start,
                        g, ba, n_types, thr, counts, row_off, nbr, (int*)nullptr);
     if (int rcode = launch_check("k_nbr_cell<fill>")) return rcode;

@@ -267,8 +267,8 @@ struct Stage {
 
 // The adjoint keeps a compact record (x, y, z, 1/r) + per group (fc, dfc / r)
 // and no per-neighbour row slot (r^2 is recomputed per pair from x, y, z,
-// edge positions come back from the staging ballots): the LDS per wave then allows three waves
-// per SIMD in FP64 and four in FP32.
+// edge positions come back from the staging ballots): the LDS per wave then
+// allows three waves per SIMD in FP64 and four and a half in FP32.
 
 template <typename R, int MODE>
 __host__ __device__ constexpr size_t stage_bytes_per_nbr() {
@@ -476,10 +476,9 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
     // one live pair (ia, ib)
     auto body = [&](const int ia, const int ib) {
         const V4<R> pa = S.P[ia], pb = S.P[ib];
-        V4<R> qa, qb;   // (r, r^2, fc, dfc)
+        V4<R> qa, qb;   // (r, r^2, fc, dfc); ADJ: (unused, r^2, fc, dfc / r)
         R ira, irb;     // 1 / r
         if constexpr ((MODE == MODE_ADJ)) {
-            // qa.w = dfc / r here
             const V2<R> ua = S.QF[ia], ub = S.QF[ib];
             ira = pa.w;
             irb = pb.w;

@@ -34,6 +34,7 @@ import os
 import sys
 import time
 from pathlib import Path
+from types import SimpleNamespace
 
 import numpy as np
 
@@ -79,14 +80,34 @@ def neighbour_stats(s, rc, sample=256, seed=0):
     return float(np.mean(n_nb)), float(np.mean(n_tr))
 
 
-def algorithmic_ops(n_nb, n_trip, g2_per_element=8, g3_per_pair=12):
-    """Per-atom ops of one descriptor forward and adjoint (transcendental = 1 op, FMA = 2).
+def algorithmic_ops(n_nb, n_trip, g2=8, g3=12, n_eta=3, quartic=True, angular_groups_per_neighbour=2):
+    """Algorithmic ops per centre of one descriptor forward and adjoint pass
+    (add/mul/compare 1, FMA 2, transcendental 1; screening, queueing, index and
+    LDS-address arithmetic are not counted).
 
-    forward: (15 + 6 g2) n_nb + (26 + 5 g3) n_trip  (63 n_nb + 86 n_trip for the teacher);
-    adjoint: 2.5x the forward.
+    forward (the per-pair math of the descriptor definitions):
+        (15 + 6 g2) n_nb + (26 + 5 g3) n_trip
+    adjoint (counted from the pair and neighbour bodies of k_acsf in adjoint mode):
+      per live pair {j, k} (r_jk < rc), all functions of its angular group:
+        geometry 31   (r_ij^2, r_ik^2 10; R_jk 3; r_jk^2 5; 1/sqrt + r 2; cos 7;
+                       S 2; 1/r_ij^2, 1/r_ik^2 2)
+        cutoff 18     (f and f' of r_jk 10; F = fc fc fc and its three partials 8)
+        angular       quartic class (zeta {1,4} x lambda +-1): shared powers of cos 8,
+                      then 18 per eta (exp 2, U 6, U' 4, P 1, the three sums 5);
+                      two-term zeta = 1 class: terms 2, then 16 per eta
+        assembly 17   (d/dr_jk, d/dcos, the two radial coefficients, 1/(r_ij r_ik) term)
+        gradient 36   (two 3-vectors of three terms 30, six accumulations 6)
+        -> 110 + 18 n_eta (quartic) or 104 + 16 n_eta (two-term)
+      per neighbour: radial group 18 + 11 g2 (r 1, f and f' 10, per function d,
+        exp, derivative and weight 11, scaling and three accumulations 7), the
+        angular groups' f and f'/r 12 each, the centre sum 3
+        -> 21 + 11 g2 + 12 * angular_groups_per_neighbour
     """
-    fwd = (15.0 + 6.0 * g2_per_element) * n_nb + (26.0 + 5.0 * g3_per_pair) * n_trip
-    return fwd, 2.5 * fwd
+    fwd = (15.0 + 6.0 * g2) * n_nb + (26.0 + 5.0 * g3) * n_trip
+    pair = (110.0 + 18.0 * n_eta) if quartic else (104.0 + 16.0 * n_eta)
+    nb = 21.0 + 11.0 * g2 + 12.0 * angular_groups_per_neighbour
+    adj = nb * n_nb + pair * n_trip
+    return fwd, adj
 
 
 def profiled_traffic(kernel: str, grid: int):
@@ -104,48 +125,127 @@ def profiled_traffic(kernel: str, grid: int):
     return float(np.mean(vals)) if vals else None
 
 
-def cpu_baseline(pot, s, budget_s=12.0):
-    """FP64 oracle (C, OpenMP) timed on a sample of centres of the same structure.
+def _oracle_threads() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
-    Per centre: dense O(N) minimum-image scan (the reference's mask), the
-    descriptor by per-function loops, scaler + MLP, and the force VJP.
-    atom-steps/s = centres / seconds.
-    """
+
+def _mlp_weights(onn, types, centres, G):
+    """w = dE_i/dG_i of the oracle's scaler + MLP for the rows G of ``centres``."""
+    from oracle import pyoracle
+
+    w = np.zeros_like(G)
+    for e in onn.elements:
+        sel = types[centres] == onn.type_of[e]
+        if not np.any(sel):
+            continue
+        x = pyoracle.scaler_transform(onn.scaler_kind, onn.scaler_params[e], G[sel], onn.smin, onn.smax)
+        _, dy = pyoracle.mlp_forward_backward(onn.layers[e], x)
+        w[sel, : dy.shape[1]] = dy * pyoracle.scaler_slope(onn.scaler_kind, onn.scaler_params[e], onn.smin,
+                                                           onn.smax)
+    return w
+
+
+def _host_water(pot, n_molecules: int, box: float, seed: int = 2023):
+    """(oracle NNP, wrapped positions, types, box) of a water box, on the host only."""
+    from types import SimpleNamespace
+
     from oracle import pyoracle
     from oracle.bridge import oracle_nnp
+    from pantea_amd import water
+    from pantea_amd.atoms import ElementMap
 
-    onn = oracle_nnp(pot, s)
-    pos = s.positions.cpu().numpy()
-    types = s.atom_types_host
-    L = s.box_lengths
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    d = water.water_box(n_molecules, box, seed=seed)
+    emap = ElementMap.from_list(d["elements"])
+    onn = oracle_nnp(pot, SimpleNamespace(element_map=emap))
+    L = np.array([box, box, box], dtype=np.float64)
+    types = np.array([emap.element_to_atom_type[e] for e in d["elements"]], dtype=np.int32)
+    return onn, pyoracle.wrap_positions(d["positions"], L), types, L
 
-    def run(k):
-        rng = np.random.default_rng(1)
-        centres = rng.choice(s.natoms, k, replace=False)
+
+def cpu_celllist(onn, pos, types, L, rc: float, n_centres: int):
+    """Cell-list CPU port (FP64 oracle, OpenMP): cell-list neighbour rows of all
+    atoms, then for a fixed sample of centres the descriptors (per-function loops
+    over the centre's neighbour pairs), scaler + MLP and the force VJP.
+    atom-steps/s = 1 / (list time per atom + centre time per centre)."""
+    from oracle import pyoracle
+
+    n = len(types)
+    t0 = time.perf_counter()
+    row_off, idx = pyoracle.cell_neighbors(pos, L, rc)
+    t_list = time.perf_counter() - t0
+    k = min(n_centres, n)
+    centres = np.sort(np.random.default_rng(1).choice(n, k, replace=False)).astype(np.int32)
+    t0 = time.perf_counter()
+    for e in onn.elements:
+        sel = centres[types[centres] == onn.type_of[e]]
+        if sel.size == 0:
+            continue
+        G, _ = pyoracle.celllist_centres(pos, types, L, row_off, idx, onn.tables[e], sel)
+        w = _mlp_weights(onn, types, sel, G)
+        pyoracle.celllist_centres(pos, types, L, row_off, idx, onn.tables[e], sel, w)
+    t_c = time.perf_counter() - t0
+    value = 1.0 / (t_list / n + t_c / k)
+    return value, {"atoms": n, "centres": k, "list_seconds": t_list, "centre_seconds": t_c}
+
+
+def cpu_reference_ladder(pot, sizes=(32, 64, 128), seed=2023):
+    """The reference's dense algorithm (all N atoms per radial function, all N x N
+    ordered pairs per angular function, every term evaluated and masked) on water
+    boxes of 3n atoms at the 128-H2O density: descriptors of every atom, scaler +
+    MLP, then the self-term force pass through the same loops (jax.grad w.r.t. the
+    central positions, force.py:16-43).  Returns per-N atom-steps/s and the fitted
+    exponent of time in N."""
+    from oracle import pyoracle
+
+    rho = 128 / 29.56 ** 3
+    out = []
+    for nm in sizes:
+        L = (nm / rho) ** (1.0 / 3.0)
+        onn, pos, types, box = _host_water(pot, nm, L, seed)
+        st = SimpleNamespace(natoms=len(types), box_lengths=box)
         t0 = time.perf_counter()
-        w = np.zeros((s.natoms, max(t.shape[0] for t in onn.tables.values())))
         for e in onn.elements:
-            sel = centres[types[centres] == onn.type_of[e]]
-            if sel.size == 0:
-                continue
-            G = pyoracle.acsf(pos, types, L, onn.tables[e], center_pos=pos[sel], self_idx=sel.astype(np.int32))
-            x = pyoracle.scaler_transform(onn.scaler_kind, onn.scaler_params[e], G, onn.smin, onn.smax)
-            _, dy = pyoracle.mlp_forward_backward(onn.layers[e], x)
-            w[sel, : dy.shape[1]] = dy * pyoracle.scaler_slope(onn.scaler_kind, onn.scaler_params[e], onn.smin, onn.smax)
-        want = np.zeros(s.natoms, np.uint8)
-        want[centres] = 1
-        pyoracle.force_vjp(pos, types, L, onn._tables_by_type(), w, want)
-        return time.perf_counter() - t0
+            sel = np.nonzero(types == onn.type_of[e])[0].astype(np.int32)
+            G, _ = pyoracle.reference_algorithm(pos, types, st.box_lengths, onn.tables[e], sel)
+            w = _mlp_weights(onn, types, sel, G)
+            pyoracle.reference_algorithm(pos, types, st.box_lengths, onn.tables[e], sel, w)
+        t = time.perf_counter() - t0
+        out.append({"atoms": st.natoms, "box_bohr": L, "seconds_per_step": t, "atom_steps_per_s": st.natoms / t})
+    n = np.array([o["atoms"] for o in out], dtype=np.float64)
+    t = np.array([o["seconds_per_step"] for o in out])
+    p = float(np.polyfit(np.log(n), np.log(t), 1)[0])
+    return out, p
 
-    k = 2 * threads
-    t = run(k)
-    k2 = int(max(k, min(20000, k * budget_s / max(t, 1e-3))))
-    t2 = run(k2)
-    return {"value": k2 / t2, "unit": "atom-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{k2} centres of the {s.natoms}-atom structure: oracle FP64 descriptor + MLP + force "
-                      f"VJP per centre with the reference algorithm (dense minimum-image scan, per-function "
-                      f"loops), {t2:.1f} s on {threads} OpenMP threads"}
+
+def cpu_baselines(pot, molecules: int, box: float, threads: int):
+    """All CPU legs: the headline object (the cell-list port on the headline
+    structure) plus the cell-list port at 128 and 4096 H2O and the
+    reference-algorithm ladder."""
+    legs = {}
+    head = None
+    for nm, L in ((128, 29.56), (4096, 93.85), (molecules, box)):
+        onn, pos, types, Lv = _host_water(pot, nm, L)
+        v, inf = cpu_celllist(onn, pos, types, Lv, pot.r_cutoff, 2048)
+        legs[f"{nm} H2O"] = {"atom_steps_per_s": v, **inf}
+        head = (v, inf)
+    value, info = head
+    cpu = {"value": value, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+           "sample": f"FP64 oracle (C, OpenMP) with a cell list on the {info['atoms']}-atom structure: neighbour rows "
+                     f"of all atoms ({info['list_seconds']:.2f} s), then descriptors (per-function loops over each "
+                     f"centre's neighbour pairs), scaler + MLP and the force VJP for a fixed sample of "
+                     f"{info['centres']} centres ({info['centre_seconds']:.2f} s); atom-steps/s = 1 / (list time per "
+                     f"atom + centre time per centre)"}
+    ladder, expo = cpu_reference_ladder(pot)
+    last = ladder[-1]
+    n_head = info["atoms"]
+    extrap = last["atom_steps_per_s"] * (last["atoms"] / n_head) ** (expo - 1.0)
+    ref = {"kind": "reference algorithm (dense: all N atoms per radial function, all N x N pairs per angular "
+                   "function, every term evaluated and masked)", "cores": threads, "ladder": ladder,
+           "time_exponent_fit": expo,
+           "extrapolated_atom_steps_per_s_at_headline_atoms": extrap,
+           "note": f"measured at {', '.join(str(o['atoms']) for o in ladder)} atoms; the value at {n_head} atoms "
+                   "is an extrapolation of the fitted power law (time ~ N^p), not a measurement"}
+    return cpu, {"celllist_port": legs, "reference_algorithm": ref}
 
 
 def _thermostat(args):

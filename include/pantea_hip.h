@@ -8,7 +8,8 @@
  *    before the call returns.
  *  - Work is enqueued asynchronously on `stream` (a hipStream_t; pass
  *    torch.cuda.current_stream().cuda_stream).  Only pt_nbr_build
- *    synchronises `stream` (once, to read the pair count back).
+ *    synchronises `stream` (once, to read the pair count back, through a
+ *    pageable copy: it allocates nothing); pt_nbr_build_async does not.
  *  - Return value: PT_OK (0) or a negative PT_ERR_*; pt_last_error() gives a
  *    thread-local message.  Buffers are owned by the caller; scratch comes in
  *    through `ws`/`ws_bytes`, sized by the *_workspace_bytes() queries.
@@ -155,6 +156,26 @@ int pt_nbr_build(int n, const double *pos, int pos_stride, const int *types, int
                  long long cap, long long *n_pairs, int *max_row,
                  void *ws, size_t ws_bytes, pt_stream_t stream);
 
+/* The same build for an MD step that never waits on the host (periodic
+ * boxes only); no counterpart in the reference, which has no neighbour list.
+ * Differences from pt_nbr_build:
+ *  - gate (device, optional): when *gate == 0 every kernel of the build
+ *    returns at once and perm, pos4, row_off, nbr and rev keep their
+ *    contents (pt_skin_check sets the gate from the skin criterion);
+ *  - nothing is read back: totals (device, 4 x int64, cleared by the caller)
+ *    get [0] the pair count of the last build, [1] the longest row and [2]
+ *    the largest pair count since they were cleared, [3] += 1 per build.
+ *    [2] > cap means a build overflowed the arrays (the list is then
+ *    incomplete); the caller checks [1] against the max_row it passes to the
+ *    descriptor kernels (a longer row turns that centre's outputs into NaN);
+ *  - rows longer than 256 entries take the second pass, decided on the device.
+ * No synchronisation and no host read: every launch is enqueued whatever the
+ * data, so the sequence can be replayed or captured. */
+int pt_nbr_build_async(int n, const double *pos, int pos_stride, const int *types, int n_types,
+                       const double *box, double rc, int *perm, double *pos4, long long *row_off,
+                       int *nbr, int *rev, long long cap, const int *gate, long long *totals,
+                       void *ws, size_t ws_bytes, pt_stream_t stream);
+
 /* Mark the closed neighbourhoods of n_sel atoms: flags[s] = 1 and
  * flags[j] = 1 for every j in the row of s (flags: device, n entries, not
  * cleared here).  Used to evaluate a potential on the centres whose
@@ -171,7 +192,9 @@ int pt_sf_table_kind(const pt_sf_table *host_table);
  *     centers[c] (or center_begin + c when centers == NULL), internal order.
  * Replaces _calculate_acsf_descriptor (descriptors/acsf/acsf.py:163-213).
  *  G (device, out): n_centers x ldg, FP64 (PT_F64) or FP32 (PT_F32).
- *  max_row: the longest neighbour row (from pt_nbr_build).
+ *  max_row: the longest neighbour row (from pt_nbr_build); it sizes each
+ *           centre's LDS staging, and a centre whose row is longer is
+ *           written as NaN (for the adjoint: f_self and the row's edges).
  * ---------------------------------------------------------------------- */
 int pt_acsf_forward(int precision, const double *pos4, const double *box,
                     const long long *row_off, const int *nbr, int max_row,
@@ -277,6 +300,31 @@ int pt_adam_step(int n, double *params, const double *grad, double *m, double *v
                  pt_stream_t stream);
 
 /* ------------------------------------------------------------------------
+ * Scaler statistics and extrapolation counter on the device
+ * (descriptors/scaler.py:84-142, 249-295).  params (device) holds 4 x n_cols
+ * FP64: mean | sigma | minval | maxval (the ScalerParams fields).  G: n_rows
+ * descriptor rows, leading dimension ldg, FP64 or FP32 (g_precision).
+ * ---------------------------------------------------------------------- */
+size_t pt_scaler_workspace_bytes(int n_rows, int n_cols);
+/* nsamples == 0: params = the statistics of G (_fit, scaler.py:249-259:
+ * mean, population std, min, max).  nsamples > 0: params (the statistics of
+ * nsamples earlier rows) are merged with those of G as _partial_fit does
+ * (scaler.py:262-283).  Deterministic (fixed reduction order). */
+int pt_scaler_partial_fit(int n_rows, int n_cols, const void *G, int ldg, int g_precision,
+                          double nsamples, double *params, void *ws, size_t ws_bytes, pt_stream_t stream);
+/* Extrapolation count (_calculate_number_of_warnings, scaler.py:286-295,
+ * which reports whether any value lies outside [minval, maxval]): a value
+ * outside [minval - margin w, maxval + margin w], w = maxval - minval, or
+ * NaN, is out of range (margin 0: the reference's test).  Accumulates into
+ * counters (device, 3 x uint64, cleared by the caller):
+ *   [0] += rows with an out-of-range value, [1] += out-of-range values,
+ *   [2] = max([2], bit pattern of the largest excess in units of w).
+ * No synchronisation: the counters can be read whenever the caller syncs. */
+int pt_scaler_extrapolation(int n_rows, int n_cols, const void *G, int ldg, int g_precision,
+                            const double *params, double margin, unsigned long long *counters,
+                            pt_stream_t stream);
+
+/* ------------------------------------------------------------------------
  * (d) Reductions and the velocity-Verlet step (FP64).
  * ---------------------------------------------------------------------- */
 size_t pt_reduce_workspace_bytes(int n);
@@ -314,6 +362,22 @@ int pt_berendsen_rescale(int n, double *vel, const double *ke, double dt, double
  * has no neighbour list). */
 int pt_max_displacement(int n, const double *pos4, const double *ref, const double *box,
                         double *out, pt_stream_t stream);
+
+/* *flag (device int) = 1 when a skin list built at ref must be rebuilt at
+ * pos4: the two largest |MIC(pos4_i - ref_i)| sum to more than skin (or skin
+ * <= 0, or a displacement is NaN), else 0.  scratch (device): 514 doubles,
+ * as the out array of pt_max_displacement.  No host read. */
+int pt_skin_check(int n, const double *pos4, const double *ref, const double *box, double skin,
+                  double *scratch, int *flag, pt_stream_t stream);
+/* dst <- src (n_words words of word_bytes = 4 or 8), unless gate (device,
+ * optional) holds 0. */
+int pt_copy_gated(long long n_words, int word_bytes, const void *src, void *dst, const int *gate,
+                  pt_stream_t stream);
+/* In place, unless gate (device, optional) holds 0: row i of buf (width
+ * words of word_bytes = 4 or 8) <- old row perm[i], through tmp (same size).
+ * Moves the MD state into the order of a gated rebuild. */
+int pt_permute_rows_gated(int n, int width, int word_bytes, const int *perm, void *buf, void *tmp,
+                          const int *gate, pt_stream_t stream);
 
 /* Row gathers/scatters between input order and internal order. */
 int pt_gather_rows_f64(int n, int width, const int *idx, const double *src, double *dst,

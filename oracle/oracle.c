@@ -211,10 +211,8 @@ static inline double pair_geom(const nb_t *a, const nb_t *b, const double *box,
  * neighbours (radial) or all ordered pairs (j, k) (angular), halving when
  * the two neighbour elements coincide (acsf.py:285-290).
  * ---------------------------------------------------------------------- */
-static void acsf_one(int n, const double *pos, const int *types, const double *box,
-                     int nsf, const double *sf, const double *ci, int self_idx,
-                     nb_t *row, double *out) {
-    int m = gather_row(n, pos, types, box, ci, self_idx, max_rc(nsf, sf), row);
+static void acsf_row(int m, const nb_t *row, const double *box, int nsf, const double *sf,
+                     double *out) {
     for (int s = 0; s < nsf; ++s) {
         const double *p = sf + SF_STRIDE * s;
         int kind = (int)p[0], ej = (int)p[1], ek = (int)p[2], ct = (int)p[8];
@@ -255,6 +253,13 @@ static void acsf_one(int n, const double *pos, const int *types, const double *b
         }
         out[s] = acc;
     }
+}
+
+static void acsf_one(int n, const double *pos, const int *types, const double *box,
+                     int nsf, const double *sf, const double *ci, int self_idx,
+                     nb_t *row, double *out) {
+    int m = gather_row(n, pos, types, box, ci, self_idx, max_rc(nsf, sf), row);
+    acsf_row(m, row, box, nsf, sf, out);
 }
 
 /* Descriptor rows for centres at explicit positions.  self_idx[c] is the
@@ -455,5 +460,262 @@ int or_nnp_forces(int n, const double *pos, const int *types, const double *box,
         }
     }
     free(part);
+    return err;
+}
+
+/* ------------------------------------------------------------------------
+ * The reference ALGORITHM, for timing the CPU baseline (not a checker: the
+ * functions above are).  Per centre i (acsf.py:163-202): MIC differences and
+ * distances to all N atoms; per symmetry function a masked sum over all N
+ * atoms (radial, acsf.py:231-246) or over all N x N ordered pairs (angular:
+ * lax.scan over j with a vector over k, acsf.py:249-330).  Like the
+ * vectorised JAX code, every (j, k) term is evaluated and the mask applied
+ * to the sum, so the angular work is O(N^2) per function and centre.
+ * With w != NULL the same loops also accumulate the gradient of
+ * sum_s w_s G_is with respect to the centre (force.py:16-43: jax.grad w.r.t.
+ * the central positions), written to grad (nc x 3).
+ * ---------------------------------------------------------------------- */
+static void dense_centre(int n, const double *pos, const int *types, const double *box,
+                         int nsf, const double *sf, int i, double *R, double *r,
+                         double *G, const double *w, double *grad) {
+    double gself[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < n; ++j) r[j] = dist3(pos + 3 * i, pos + 3 * j, box, R + 3 * j);
+    for (int s = 0; s < nsf; ++s) {
+        const double *p = sf + SF_STRIDE * s;
+        int kind = (int)p[0], ej = (int)p[1], ek = (int)p[2], ct = (int)p[8];
+        double eta = p[3], rs = p[4], zeta = p[5], lam = p[6], rc = p[7];
+        double acc = 0.0;
+        double ws = w ? w[s] : 0.0;
+        if (kind == 1 || kind == 2) {
+            for (int j = 0; j < n; ++j) {
+                double rj = r[j];
+                double f = fc_val(ct, rc, rj);
+                double ex = (kind == 1) ? 1.0 : exp(-eta * ((rj - rs) * (rj - rs)));
+                int mask = (rj <= rc) && (rj > 0.0) && types[j] == ej;
+                acc += mask ? ex * f : 0.0;
+                if (w) {
+                    double fp = fc_der(ct, rc, rj);
+                    double dg = (kind == 1) ? fp : ex * (fp - 2.0 * eta * (rj - rs) * f);
+                    double sc = mask ? ws * dg / rj : 0.0;
+                    for (int q = 0; q < 3; ++q) gself[q] += sc * R[3 * j + q];
+                }
+            }
+        } else {
+            double pre = pow(2.0, 1.0 - zeta);
+            for (int j = 0; j < n; ++j) {
+                double rj = r[j];
+                int mj = (rj <= rc) && (rj > 0.0) && types[j] == ej;
+                double fij = fc_val(ct, rc, rj), dfij = w ? fc_der(ct, rc, rj) : 0.0;
+                double v = 0.0;
+                for (int k = 0; k < n; ++k) {
+                    double rk = r[k];
+                    int mk = (rk <= rc) && (rk > 0.0) && types[k] == ek;
+                    double operand = rj * rk;
+                    double dot = (R[3 * j] * R[3 * k] + R[3 * j + 1] * R[3 * k + 1]) + R[3 * j + 2] * R[3 * k + 2];
+                    double cost = mk ? (operand == 0.0 ? 0.0 : dot / operand) : 1.0;
+                    double Rjk[3];
+                    double rjk = mk ? dist3(R + 3 * j, R + 3 * k, box, Rjk) : 0.0;
+                    if (!mk) Rjk[0] = Rjk[1] = Rjk[2] = 0.0;
+                    double fik = fc_val(ct, rc, rk);
+                    double fjk = (kind == 3) ? fc_val(ct, rc, rjk) : 1.0;
+                    double S = (kind == 3) ? rj * rj + rk * rk + rjk * rjk : rj * rj + rk * rk;
+                    double X = exp(-eta * S);
+                    double base = 1.0 + lam * cost;
+                    double P = pow(base, zeta);
+                    double t = pre * P * X * fij * fik * fjk;
+                    int m = mk && (rjk > 0.0);
+                    v += m ? t : 0.0;
+                    if (w && mj && m) {
+                        double dfik = fc_der(ct, rc, rk);
+                        double dP = (zeta == 0.0) ? 0.0 : zeta * lam * pow(base, zeta - 1.0);
+                        double W = ws * pre * ((ej == ek) ? 0.5 : 1.0);
+                        double dT_dc = W * dP * X * fij * fik * fjk;
+                        double dT_drij = W * P * X * fik * fjk * (dfij - 2.0 * eta * rj * fij);
+                        double dT_drik = W * P * X * fij * fjk * (dfik - 2.0 * eta * rk * fik);
+                        double inv = 1.0 / (rj * rk);
+                        for (int q = 0; q < 3; ++q) {
+                            double dc_dRij = R[3 * k + q] * inv - cost * R[3 * j + q] / (rj * rj);
+                            double dc_dRik = R[3 * j + q] * inv - cost * R[3 * k + q] / (rk * rk);
+                            /* R_jk = R_ij - R_ik does not move with r_i */
+                            gself[q] += dT_drij * R[3 * j + q] / rj + dT_drik * R[3 * k + q] / rk +
+                                        dT_dc * (dc_dRij + dc_dRik);
+                        }
+                    }
+                }
+                acc += mj ? v : 0.0;
+            }
+            if (ej == ek) acc = acc * 0.5;
+        }
+        if (G) G[s] = acc;
+    }
+    if (grad) for (int q = 0; q < 3; ++q) grad[q] = gself[q];
+}
+
+int or_reference_algorithm(int n, const double *pos, const int *types, const double *box,
+                           int nsf, const double *sf, int nc, const int *centres,
+                           double *G, const double *w, double *grad) {
+    int err = 0;
+#pragma omp parallel
+    {
+        double *R = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+        double *r = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+        if (!R || !r) {
+#pragma omp atomic write
+            err = -1;
+        } else {
+#pragma omp for schedule(dynamic, 1)
+            for (int c = 0; c < nc; ++c)
+                dense_centre(n, pos, types, box, nsf, sf, centres[c], R, r,
+                             G ? G + (size_t)nsf * c : NULL, w ? w + (size_t)nsf * c : NULL,
+                             grad ? grad + 3 * (size_t)c : NULL);
+        }
+        free(R); free(r);
+    }
+    return err;
+}
+
+/* ------------------------------------------------------------------------
+ * Cell-list neighbour rows (same rows as or_neighbors: { j : r_ij <= rc,
+ * r_ij > 0 } under the single-shift minimum image, ascending j) in O(N) for
+ * periodic boxes with at least 3 cells of width >= rc per dimension; other
+ * boxes fall back to the dense scan.  For timing the cell-list CPU baseline.
+ * ---------------------------------------------------------------------- */
+static int cmp_int(const void *a, const void *b) {
+    int x = *(const int *)a, y = *(const int *)b;
+    return (x > y) - (x < y);
+}
+
+long or_cell_neighbors(int n, const double *pos, const double *box, double rc,
+                       long long *row_off, int *idx, long cap) {
+    int nc[3];
+    int ok = box != NULL;
+    for (int d = 0; d < 3 && ok; ++d) {
+        nc[d] = (int)floor(box[d] / rc);
+        if (nc[d] < 3) ok = 0;
+    }
+    if (!ok) {
+        int *counts = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+        if (!counts) return 0;
+        long t = or_neighbors(n, pos, box, rc, counts, idx, cap);
+        row_off[0] = 0;
+        for (int i = 0; i < n; ++i) row_off[i + 1] = row_off[i] + counts[i];
+        free(counts);
+        return t;
+    }
+    long ncell = (long)nc[0] * nc[1] * nc[2];
+    int *head = (int *)malloc(sizeof(int) * (size_t)(ncell + 1));
+    int *order = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int *cell = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int *cnt = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
+    if (!head || !order || !cell || !cnt) { free(head); free(order); free(cell); free(cnt); return 0; }
+    for (long c = 0; c <= ncell; ++c) head[c] = 0;
+    for (int i = 0; i < n; ++i) {
+        int ci[3];
+        for (int d = 0; d < 3; ++d) {
+            double x = wrap1(pos[3 * i + d], box[d]);
+            int k = (int)(x / box[d] * nc[d]);
+            ci[d] = k < 0 ? 0 : (k >= nc[d] ? nc[d] - 1 : k);
+        }
+        cell[i] = (ci[0] * nc[1] + ci[1]) * nc[2] + ci[2];
+        head[cell[i] + 1]++;
+    }
+    for (long c = 0; c < ncell; ++c) head[c + 1] += head[c];
+    {
+        int *fill = (int *)malloc(sizeof(int) * (size_t)ncell);
+        if (!fill) { free(head); free(order); free(cell); free(cnt); return 0; }
+        for (long c = 0; c < ncell; ++c) fill[c] = head[c];
+        for (int i = 0; i < n; ++i) order[fill[cell[i]]++] = i;
+        free(fill);
+    }
+    /* count, then fill each row (sorted ascending) */
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            row_off[0] = 0;
+            for (int i = 0; i < n; ++i) row_off[i + 1] = row_off[i] + cnt[i];
+            if (row_off[n] > cap) break;
+        }
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int i = 0; i < n; ++i) {
+            int c0 = cell[i];
+            int cz = c0 % nc[2], cy = (c0 / nc[2]) % nc[1], cx = c0 / (nc[1] * nc[2]);
+            int m = 0;
+            int *out = pass ? idx + row_off[i] : NULL;
+            for (int dx = -1; dx <= 1; ++dx)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dz = -1; dz <= 1; ++dz) {
+                        int c = (((cx + dx + nc[0]) % nc[0]) * nc[1] + (cy + dy + nc[1]) % nc[1]) * nc[2] +
+                                (cz + dz + nc[2]) % nc[2];
+                        for (int q = head[c]; q < head[c + 1]; ++q) {
+                            int j = order[q];
+                            double d3[3];
+                            double r = dist3(pos + 3 * i, pos + 3 * j, box, d3);
+                            if (r <= rc && r > 0.0) {
+                                if (pass) out[m] = j;
+                                ++m;
+                            }
+                        }
+                    }
+            if (pass) qsort(out, (size_t)m, sizeof(int), cmp_int);
+            else cnt[i] = m;
+        }
+    }
+    long total = 0;
+    for (int i = 0; i < n; ++i) total += cnt[i];
+    free(head); free(order); free(cell); free(cnt);
+    return total <= cap ? total : -total;
+}
+
+/* Row of centre i from CSR neighbour rows (the masked entries of the dense
+ * row, in the same order). */
+static int csr_row(const double *pos, const int *types, const double *box, int i,
+                   const long long *row_off, const int *idx, nb_t *row) {
+    int m = 0;
+    for (long long e = row_off[i]; e < row_off[i + 1]; ++e) {
+        int j = idx[e];
+        double dx[3];
+        double r = dist3(pos + 3 * i, pos + 3 * j, box, dx);
+        row[m].j = j;
+        row[m].type = types[j];
+        row[m].R[0] = dx[0]; row[m].R[1] = dx[1]; row[m].R[2] = dx[2];
+        row[m].r = r;
+        ++m;
+    }
+    return m;
+}
+
+/* Descriptors, then (with w) the per-centre VJP, of the centres `centres`
+ * from CSR rows: the cell-list CPU baseline (same arithmetic as or_acsf /
+ * or_nnp_forces).  G: nc x nsf; self_grad: nc x 3 (sum_s w_s dG_is/dr_i). */
+int or_celllist_centres(int n, const double *pos, const int *types, const double *box,
+                        const long long *row_off, const int *idx, int nsf, const double *sf,
+                        int nc, const int *centres, double *G, const double *w, double *self_grad) {
+    int err = 0;
+    int maxrow = 0;
+    for (int c = 0; c < nc; ++c) {
+        int len = (int)(row_off[centres[c] + 1] - row_off[centres[c]]);
+        if (len > maxrow) maxrow = len;
+    }
+#pragma omp parallel
+    {
+        nb_t *row = (nb_t *)malloc(sizeof(nb_t) * (size_t)(maxrow > 0 ? maxrow : 1));
+        double *d = (double *)malloc(sizeof(double) * 3 * (size_t)(maxrow > 0 ? maxrow : 1));
+        if (!row || !d) {
+#pragma omp atomic write
+            err = -1;
+        } else {
+#pragma omp for schedule(dynamic, 4)
+            for (int c = 0; c < nc; ++c) {
+                int i = centres[c];
+                int m = csr_row(pos, types, box, i, row_off, idx, row);
+                if (G) acsf_row(m, row, box, nsf, sf, G + (size_t)nsf * c);
+                if (w && self_grad) {
+                    double self[3];
+                    acsf_vjp_one(m, row, box, nsf, sf, w + (size_t)nsf * c, self, d);
+                    for (int q = 0; q < 3; ++q) self_grad[3 * (size_t)c + q] = self[q];
+                }
+            }
+        }
+        free(row); free(d);
+    }
     return err;
 }

@@ -48,6 +48,7 @@ MASS_U = {"H": 1.008, "He": 4.003, "O": 15.999, "Ne": 20.180}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_longlong)
 
 
 def _load() -> ctypes.CDLL:
@@ -62,6 +63,14 @@ def _load() -> ctypes.CDLL:
     lib.or_acsf.argtypes = [ctypes.c_int, _dp, _ip, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, _ip, _dp]
     lib.or_acsf_selfjac.restype = ctypes.c_int
     lib.or_acsf_selfjac.argtypes = lib.or_acsf.argtypes
+    lib.or_cell_neighbors.restype = ctypes.c_long
+    lib.or_cell_neighbors.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double, _lp, _ip, ctypes.c_long]
+    lib.or_celllist_centres.restype = ctypes.c_int
+    lib.or_celllist_centres.argtypes = [ctypes.c_int, _dp, _ip, _dp, _lp, _ip, ctypes.c_int, _dp, ctypes.c_int,
+                                        _ip, _dp, _dp, _dp]
+    lib.or_reference_algorithm.restype = ctypes.c_int
+    lib.or_reference_algorithm.argtypes = [ctypes.c_int, _dp, _ip, _dp, ctypes.c_int, _dp, ctypes.c_int, _ip,
+                                           _dp, _dp, _dp]
     lib.or_nnp_forces.restype = ctypes.c_int
     lib.or_nnp_forces.argtypes = [
         ctypes.c_int, _dp, _ip, _dp, ctypes.c_int, _ip, _ip, _dp,
@@ -108,6 +117,8 @@ def _p(a: np.ndarray):
         return a.ctypes.data_as(_dp)
     if a.dtype == np.int32:
         return a.ctypes.data_as(_ip)
+    if a.dtype == np.int64:
+        return a.ctypes.data_as(_lp)
     if a.dtype == np.uint8:
         return a.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
     raise TypeError(a.dtype)
@@ -189,6 +200,57 @@ def acsf(pos, types, box, sf_rows, center_pos=None, self_idx=None) -> np.ndarray
     if rc != 0:
         raise MemoryError("oracle acsf failed")
     return out
+
+
+def cell_neighbors(pos, box, rc: float) -> Tuple[np.ndarray, np.ndarray]:
+    """(row_off, idx): the rows of neighbors() built with a cell list (O(N))."""
+    pos = _f64(pos)
+    n = pos.shape[0]
+    row_off = np.zeros(n + 1, dtype=np.int64)
+    cap = max(1024, 200 * n)
+    while True:
+        idx = np.zeros(cap, dtype=np.int32)
+        t = lib().or_cell_neighbors(n, _p(pos), _p(_box(box)), float(rc), _p(row_off), _p(idx), cap)
+        if t >= 0:
+            return row_off, idx[:t]
+        cap = -t + 1024
+
+
+def celllist_centres(pos, types, box, row_off, idx, sf_rows, centres, w=None):
+    """Descriptors (and with w the self gradient sum_s w_s dG_is/dr_i) of ``centres`` from CSR rows."""
+    pos = _f64(pos)
+    types = _i32(types)
+    sf = _f64(np.asarray(sf_rows, dtype=np.float64).reshape(-1, 10))
+    cen = _i32(centres)
+    G = np.zeros((cen.size, sf.shape[0]))
+    grad = np.zeros((cen.size, 3)) if w is not None else None
+    wp = None if w is None else _f64(np.asarray(w, dtype=np.float64).reshape(cen.size, sf.shape[0]))
+    rc = lib().or_celllist_centres(pos.shape[0], _p(pos), _p(types), _p(_box(box)),
+                                   _p(np.ascontiguousarray(row_off, dtype=np.int64)),
+                                   _p(np.ascontiguousarray(idx, dtype=np.int32)), sf.shape[0], _p(sf), cen.size,
+                                   _p(cen), _p(G), _p(wp), _p(grad))
+    if rc != 0:
+        raise MemoryError("oracle cell-list centres failed")
+    return G, grad
+
+
+def reference_algorithm(pos, types, box, sf_rows, centres, w=None):
+    """Descriptors (and, with w, the gradient of sum_s w_s G_is w.r.t. each
+    centre) by the reference's dense algorithm: all N atoms per radial
+    function and all N x N ordered pairs per angular function, every term
+    evaluated and masked (acsf.py:163-330).  Used to time the CPU baseline."""
+    pos = _f64(pos)
+    types = _i32(types)
+    sf = _f64(np.asarray(sf_rows, dtype=np.float64).reshape(-1, 10))
+    cen = _i32(centres)
+    G = np.zeros((cen.size, sf.shape[0]))
+    grad = np.zeros((cen.size, 3)) if w is not None else None
+    wp = None if w is None else _f64(np.asarray(w, dtype=np.float64).reshape(cen.size, sf.shape[0]))
+    rc = lib().or_reference_algorithm(pos.shape[0], _p(pos), _p(types), _p(_box(box)), sf.shape[0], _p(sf),
+                                      cen.size, _p(cen), _p(G), _p(wp), _p(grad))
+    if rc != 0:
+        raise MemoryError("oracle reference algorithm failed")
+    return G, grad
 
 
 def acsf_selfjac(pos, types, box, sf_rows, center_pos=None, self_idx=None) -> np.ndarray:

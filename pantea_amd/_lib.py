@@ -97,7 +97,12 @@ _SIGS = {
     "pt_nbr_workspace_bytes": (_c_size, [_c_int, _c_int]),
     "pt_nbr_build": (_c_int, [_c_int, _vp, _c_int, _vp, _c_int, _vp, _c_dbl, _vp, _vp, _vp, _vp, _vp,
                               _c_ll, ctypes.POINTER(_c_ll), ctypes.POINTER(_c_int), _vp, _c_size, _vp]),
+    "pt_nbr_build_async": (_c_int, [_c_int, _vp, _c_int, _vp, _c_int, _vp, _c_dbl, _vp, _vp, _vp, _vp, _vp,
+                                    _c_ll, _vp, _vp, _vp, _c_size, _vp]),
     "pt_nbr_mark_rows": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
+    "pt_skin_check": (_c_int, [_c_int, _vp, _vp, _vp, _c_dbl, _vp, _vp, _vp]),
+    "pt_copy_gated": (_c_int, [_c_ll, _c_int, _vp, _vp, _vp, _vp]),
+    "pt_permute_rows_gated": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     "pt_sf_table_kind": (_c_int, [ctypes.POINTER(SFTable)]),
     "pt_acsf_forward": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp,
                                  _vp, _c_int, _vp]),
@@ -110,6 +115,9 @@ _SIGS = {
     "pt_mlp_pack": (_c_int, [_vp, _vp, _vp, _vp]),
     "pt_mlp_energy_grad": (_c_int, [_c_int, _c_int, _vp, _c_int, _c_int, _vp, _vp, _vp, _vp,
                                     _vp, _vp]),
+    "pt_scaler_workspace_bytes": (_c_size, [_c_int, _c_int]),
+    "pt_scaler_partial_fit": (_c_int, [_c_int, _c_int, _vp, _c_int, _c_int, _c_dbl, _vp, _vp, _c_size, _vp]),
+    "pt_scaler_extrapolation": (_c_int, [_c_int, _c_int, _vp, _c_int, _c_int, _vp, _c_dbl, _vp, _vp]),
     "pt_reduce_workspace_bytes": (_c_size, [_c_int]),
     "pt_reduce_sum_f64": (_c_int, [_c_int, _vp, _vp, _vp, _c_size, _vp]),
     "pt_verlet_positions": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_dbl, _vp, _vp]),

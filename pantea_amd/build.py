@@ -35,7 +35,7 @@ HIPCC_FLAGS = [
     f"-I{CSRC}",
 ]
 
-SOURCES = ["md.hip", "nbr.hip", "acsf.hip", "mlp.hip", "train.hip"]
+SOURCES = ["md.hip", "nbr.hip", "acsf.hip", "mlp.hip", "scaler.hip"]
 
 
 def hipcc() -> str:
@@ -46,7 +46,7 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> Path:
-    srcs = [str(CSRC / s) for s in SOURCES if (CSRC / s).exists()]
+    srcs = [str(CSRC / s) for s in SOURCES]
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc(), *HIPCC_FLAGS, *srcs, "-o", str(tmp)]
     if verbose:

@@ -739,6 +739,22 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
     const double4 pi = reinterpret_cast<const double4*>(a.pos4)[i];
     const long long off = a.row_off[i];
     const int len = (int)(a.row_off[i + 1] - off);
+    if (len > cap) {
+        // the row does not fit the staging slice sized by the caller's max_row:
+        // stage nothing and report the centre as NaN (outputs, f_self, the row's edges)
+        const R nan = __builtin_nan("");
+        if (MODE != MODE_ADJ) {
+            R* out = (R*)a.out;
+            for (int q = lane; q < a.ld; q += kWave) out[(size_t)c * a.ld + q] = nan;
+        } else {
+            if (a.edge_grad) {
+                R* eg = (R*)a.edge_grad + 3 * off;
+                for (int t = lane; t < 3 * len; t += kWave) eg[t] = nan;
+            }
+            if (a.f_self && lane < 3) a.f_self[3 * c + lane] = (double)nan;
+        }
+        return;
+    }
     const double rcmax = tab->rc_max;
     const R tanh_pre = (R)tab->tanh_pre;
     // |R_ij - R_ik| <= 2 rc_max: the image shift can only act when L/2 < 2 rc_max

@@ -242,6 +242,33 @@ __global__ __launch_bounds__(256) void k_force_gather(int n, const long long* __
 
 inline int nblocks(long long work, int threads) { return (int)((work + threads - 1) / threads); }
 
+// ---- gated state updates of an MD step that never waits on the host ----------
+// *flag = 1 when the list must be rebuilt: the two largest displacements since
+// the last build sum to more than the skin (a NaN displacement rebuilds too).
+__global__ void k_skin_flag(const double* __restrict__ d, double skin, int* __restrict__ flag) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = (skin > 0.0 && d[0] + d[1] <= skin) ? 0 : 1;
+}
+
+__device__ __forceinline__ bool gate_closed(const int* gate) { return gate != nullptr && *gate == 0; }
+
+template <typename T>
+__global__ void k_copy_gated(const int* __restrict__ gate, long long n, const T* __restrict__ src,
+                             T* __restrict__ dst) {
+    if (gate_closed(gate)) return;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        dst[t] = src[t];
+}
+
+template <typename T>
+__global__ void k_gather_gated(const int* __restrict__ gate, int n, int width, const int* __restrict__ idx,
+                               const T* __restrict__ src, T* __restrict__ dst) {
+    if (gate_closed(gate)) return;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * width) return;
+    const int i = (int)(t / width), k = (int)(t - (long long)i * width);
+    dst[t] = src[(long long)idx[i] * width + k];
+}
+
 }  // namespace
 }  // namespace pt
 
@@ -335,6 +362,46 @@ extern "C" int pt_max_displacement(int n, const double* pos4, const double* ref,
     hipLaunchKernelGGL(k_max_disp_final, dim3(1), dim3(256), 0, (hipStream_t)stream, nb,
                        (const double*)(out + 2), out);
     return launch_check("pt_max_displacement");
+}
+
+extern "C" int pt_skin_check(int n, const double* pos4, const double* ref, const double* box, double skin,
+                             double* scratch, int* flag, pt_stream_t stream) {
+    PT_REQUIRE(n >= 0 && pos4 && ref && scratch && flag, "pt_skin_check: bad arguments");
+    if (int rc = pt_max_displacement(n, pos4, ref, box, scratch, stream)) return rc;
+    hipLaunchKernelGGL(k_skin_flag, dim3(1), dim3(64), 0, (hipStream_t)stream, (const double*)scratch, skin, flag);
+    return launch_check("pt_skin_check");
+}
+
+extern "C" int pt_copy_gated(long long n_words, int word_bytes, const void* src, void* dst, const int* gate,
+                             pt_stream_t stream) {
+    PT_REQUIRE(n_words >= 0 && (word_bytes == 4 || word_bytes == 8) && (n_words == 0 || (src && dst)),
+               "pt_copy_gated: bad arguments");
+    if (n_words == 0) return PT_OK;
+    const int blocks = (int)std::min<long long>(nblocks(n_words, 256), 4096);
+    if (word_bytes == 8)
+        hipLaunchKernelGGL(k_copy_gated<double>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, gate, n_words,
+                           (const double*)src, (double*)dst);
+    else
+        hipLaunchKernelGGL(k_copy_gated<int>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, gate, n_words,
+                           (const int*)src, (int*)dst);
+    return launch_check("pt_copy_gated");
+}
+
+extern "C" int pt_permute_rows_gated(int n, int width, int word_bytes, const int* perm, void* buf, void* tmp,
+                                     const int* gate, pt_stream_t stream) {
+    PT_REQUIRE(n >= 0 && width > 0 && (word_bytes == 4 || word_bytes == 8) && (n == 0 || (perm && buf && tmp)),
+               "pt_permute_rows_gated: bad arguments");
+    PT_REQUIRE(buf != tmp, "pt_permute_rows_gated: buf and tmp must differ");
+    if (n == 0) return PT_OK;
+    const long long words = (long long)n * width;
+    if (word_bytes == 8)
+        hipLaunchKernelGGL(k_gather_gated<double>, dim3(nblocks(words, 256)), dim3(256), 0, (hipStream_t)stream,
+                           gate, n, width, perm, (const double*)buf, (double*)tmp);
+    else
+        hipLaunchKernelGGL(k_gather_gated<int>, dim3(nblocks(words, 256)), dim3(256), 0, (hipStream_t)stream,
+                           gate, n, width, perm, (const int*)buf, (int*)tmp);
+    if (int rc = launch_check("pt_permute_rows_gated")) return rc;
+    return pt_copy_gated(words, word_bytes, tmp, buf, gate, stream);
 }
 
 extern "C" int pt_gather_rows_f64(int n, int width, const int* idx, const double* src, double* dst,

@@ -94,8 +94,14 @@ __global__ void k_bbox(int n, const double* pos, int ps, double* out) {
         for (int d = 0; d < 3; ++d) { out[d] = smin[d][0]; out[3 + d] = smax[d][0]; }
 }
 
-__global__ void k_bin(int n, const double* pos, int ps, const int* types, Grid g, int* keys, int* vals,
-                      unsigned long long* bmax) {
+// A build gated by a device flag (the skin check of an MD step) does nothing
+// when the flag is 0: every kernel of the build returns at once, so the list,
+// the order and the positions stay those of the last build.
+__device__ __forceinline__ bool gated_off(const int* gate) { return gate != nullptr && *gate == 0; }
+
+__global__ void k_bin(const int* __restrict__ gate, int n, const double* pos, int ps, const int* types, Grid g,
+                      int* keys, int* vals, unsigned long long* bmax) {
+    if (gated_off(gate)) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (i == 0) *bmax = 0ull;   // max |coordinate|, raised by k_gather_pos4
@@ -109,8 +115,9 @@ __global__ void k_bin(int n, const double* pos, int ps, const int* types, Grid g
 
 // ---- stable LSD radix sort, 8-bit digits -----------------------------------
 
-__global__ __launch_bounds__(kSortThreads) void k_radix_hist(int n, const int* keys, int shift,
-                                                             int nblocks, int* ghist) {
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const int* __restrict__ gate, int n, const int* keys,
+                                                             int shift, int nblocks, int* ghist) {
+    if (gated_off(gate)) return;
     __shared__ int hist[256];
     hist[threadIdx.x] = 0;
     __syncthreads();
@@ -126,7 +133,8 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(int n, const int* k
 // Exclusive scan of m ints (up to a few 1e5) in place by one block: each
 // thread scans a contiguous chunk of ceil(m / 1024), the chunk totals are
 // scanned across the block (wave shuffles, then the 16 wave totals).
-__global__ __launch_bounds__(1024) void k_scan_small(int m, int* data) {
+__global__ __launch_bounds__(1024) void k_scan_small(const int* __restrict__ gate, int m, int* data) {
+    if (gated_off(gate)) return;
     __shared__ int wtot[1024 / kWave];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x >> 6;
@@ -162,10 +170,11 @@ __global__ __launch_bounds__(1024) void k_scan_small(int m, int* data) {
     }
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(int n, const int* keys_in,
-                                                                const int* vals_in, int shift,
-                                                                int nblocks, const int* gscan,
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const int* __restrict__ gate, int n,
+                                                                const int* keys_in, const int* vals_in,
+                                                                int shift, int nblocks, const int* gscan,
                                                                 int* keys_out, int* vals_out) {
+    if (gated_off(gate)) return;
     __shared__ int run[256];
     __shared__ int wcount[kSortThreads / kWave][256];
     const int lane = threadIdx.x & (kWave - 1);
@@ -204,7 +213,8 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(int n, const int
     }
 }
 
-__global__ void k_cell_start(int nkeys, int n, const int* skeys, int* cell_start) {
+__global__ void k_cell_start(const int* __restrict__ gate, int nkeys, int n, const int* skeys, int* cell_start) {
+    if (gated_off(gate)) return;
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k > nkeys) return;
     int lo = 0, hi = n;
@@ -218,10 +228,12 @@ __global__ void k_cell_start(int nkeys, int n, const int* skeys, int* cell_start
 // Internal-order positions: FP64 (x, y, z, type) and an FP32 copy for the
 // candidate screen; bmax gets the bit pattern of max |coordinate| (the bit
 // patterns of non-negative doubles order like the values).
-__global__ __launch_bounds__(256) void k_gather_pos4(int n, const int* perm, const double* pos, int ps,
-                                                     const int* types, double* pos4, float4* pos4f,
-                                                     unsigned long long* bmax) {
+__global__ __launch_bounds__(256) void k_gather_pos4(const int* __restrict__ gate, int n, const int* perm,
+                                                     const double* pos, int ps, const int* types, double* pos4,
+                                                     float4* pos4f, unsigned long long* bmax, int* perm_out) {
+    if (gated_off(gate)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) perm_out[i] = perm[i];
     double m = 0.0;
     if (i < n) {
         const int v = perm[i];
@@ -323,7 +335,9 @@ constexpr int kCand = 1536;   // candidates of one cell staged in LDS (a multipl
 // search over the ranges).  Rows come out in candidate order: type-major,
 // stencil cells ascending, i.e. ascending internal index.
 template <int PASS>
-__global__ __launch_bounds__(kCellThreads) void k_nbr_cell(const double* __restrict__ pos4,
+__global__ __launch_bounds__(kCellThreads) void k_nbr_cell(const int* __restrict__ gate,
+                                                  const long long* __restrict__ total_and_max, long long cap,
+                                                  const double* __restrict__ pos4,
                                                   const float4* __restrict__ pos4f,
                                                   const unsigned long long* __restrict__ bmax,
                                                   const int* __restrict__ cell_start, Grid g,
@@ -333,8 +347,12 @@ __global__ __launch_bounds__(kCellThreads) void k_nbr_cell(const double* __restr
                                                   int* __restrict__ out_idx,
                                                   int* __restrict__ heads) {
     // out_idx: the CSR neighbour array (FILL) or the n x kStageW stage (STAGE);
-    // heads (STAGE): the first entry of each bucket of kBucket staged entries
+    // heads (STAGE): the first entry of each bucket of kBucket staged entries.
+    // FILL with total_and_max set (a launch the host did not decide on): only
+    // when some row is longer than the stage and the list fits the arrays.
     constexpr bool FILL = PASS == PASS_FILL;
+    if (gated_off(gate)) return;
+    if (FILL && total_and_max && (total_and_max[1] <= kStageW || total_and_max[0] > cap)) return;
     __shared__ int4 s_cand[kCand];            // (x, y, z) as FP32 bits, internal index
     __shared__ int s_cells[32];
     __shared__ int s_runf[32], s_runl[32];    // first / last cell of each run
@@ -514,8 +532,9 @@ __global__ __launch_bounds__(kCellThreads) void k_nbr_cell(const double* __restr
 
 // ---- row offsets: exclusive scan of counts (int64) + max row ---------------
 
-__global__ __launch_bounds__(256) void k_scan_reduce(int n, const int* counts, long long* psum,
-                                                     int* pmax) {
+__global__ __launch_bounds__(256) void k_scan_reduce(const int* __restrict__ gate, int n, const int* counts,
+                                                     long long* psum, int* pmax) {
+    if (gated_off(gate)) return;
     __shared__ long long ss[256];
     __shared__ int sm[256];
     long long s = 0;
@@ -537,8 +556,9 @@ __global__ __launch_bounds__(256) void k_scan_reduce(int n, const int* counts, l
     if (threadIdx.x == 0) { psum[blockIdx.x] = ss[0]; pmax[blockIdx.x] = sm[0]; }
 }
 
-__global__ __launch_bounds__(1024) void k_scan_top(int nb, long long* psum, const int* pmax,
-                                                   long long* total_and_max) {
+__global__ __launch_bounds__(1024) void k_scan_top(const int* __restrict__ gate, int nb, long long* psum,
+                                                   const int* pmax, long long* total_and_max) {
+    if (gated_off(gate)) return;
     __shared__ long long buf[1024];
     __shared__ long long carry;
     __shared__ int mx[1024];
@@ -571,9 +591,10 @@ __global__ __launch_bounds__(1024) void k_scan_top(int nb, long long* psum, cons
     if (threadIdx.x == 0) { total_and_max[0] = carry; total_and_max[1] = mx[0]; }
 }
 
-__global__ __launch_bounds__(256) void k_scan_down(int n, const int* counts, const long long* psum,
-                                                   const long long* total_and_max,
+__global__ __launch_bounds__(256) void k_scan_down(const int* __restrict__ gate, int n, const int* counts,
+                                                   const long long* psum, const long long* total_and_max,
                                                    long long* row_off) {
+    if (gated_off(gate)) return;
     __shared__ long long buf[256];
     constexpr int kPer = kScanBlock / 256;
     int base = blockIdx.x * kScanBlock + threadIdx.x * kPer;
@@ -602,8 +623,13 @@ __global__ __launch_bounds__(256) void k_scan_down(int n, const int* counts, con
 
 // ---- reverse edge index: rev[e(i->j)] = e(j->i) ------------------------------
 
-__global__ __launch_bounds__(256) void k_rev(int n, const long long* __restrict__ row_off,
+__global__ __launch_bounds__(256) void k_rev(const int* __restrict__ gate,
+                                             const long long* __restrict__ total_and_max, long long cap, int n,
+                                             const long long* __restrict__ row_off,
                                              const int* __restrict__ nbr, int* __restrict__ rev) {
+    // total_and_max set: a launch the host did not decide on (see k_nbr_cell)
+    if (gated_off(gate)) return;
+    if (total_and_max && (total_and_max[1] <= kStageW || total_and_max[0] > cap)) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int i = uniform(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (i >= n) return;
@@ -643,7 +669,8 @@ __device__ __forceinline__ bool has_eq(const int4& v, int base, int lim, int key
            (base + 3 < lim && v.w == key);
 }
 
-__global__ __launch_bounds__(256) void k_compact_rev(int n, const long long* __restrict__ row_off,
+__global__ __launch_bounds__(256) void k_compact_rev(const int* __restrict__ gate, int n,
+                                                     const long long* __restrict__ row_off,
                                                      const int* __restrict__ counts,
                                                      const int* __restrict__ stage,
                                                      const int* __restrict__ heads, int* __restrict__ nbr,
@@ -651,6 +678,7 @@ __global__ __launch_bounds__(256) void k_compact_rev(int n, const long long* __r
                                                      const long long* __restrict__ total_and_max,
                                                      long long cap) {
     static_assert(kBucket == 16 && kStageW % kBucket == 0, "four int4 per bucket");
+    if (gated_off(gate)) return;
     // launched before the host has read the totals: nothing to do unless the
     // list fits the arrays and every row was staged whole
     if (total_and_max[0] > cap || total_and_max[1] > kStageW) return;
@@ -806,23 +834,40 @@ extern "C" size_t pt_nbr_workspace_bytes(int n, int n_types) {
     return ws_layout(n, n_types).total;
 }
 
-extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int* types, int n_types,
-                            const double* box, double rc, int* perm, double* pos4,
-                            long long* row_off, int* nbr, int* rev, long long cap,
-                            long long* n_pairs, int* max_row, void* ws, size_t ws_bytes,
-                            pt_stream_t stream_) {
-    hipStream_t stream = (hipStream_t)stream_;
-    PT_REQUIRE(n >= 1, "pt_nbr_build: n must be >= 1 (got %d)", n);
-    PT_REQUIRE(n_types >= 1 && n_types <= kMaxTypes, "pt_nbr_build: n_types must be in [1, %d]",
-               kMaxTypes);
-    PT_REQUIRE(rc > 0.0 && std::isfinite(rc), "pt_nbr_build: rc must be finite and > 0");
-    PT_REQUIRE(pos && types && perm && pos4 && row_off && n_pairs && max_row && ws,
-               "pt_nbr_build: null pointer argument");
-    PT_REQUIRE(pos_stride >= 3, "pt_nbr_build: pos_stride must be >= 3");
-    PT_REQUIRE((const void*)pos != (const void*)pos4, "pt_nbr_build: pos and pos4 must not alias");
+namespace pt {
+namespace {
+
+// totals of a gated build: [0] pairs of the last build, [1] largest row and
+// [2] largest pair count since the caller cleared them, [3] builds done
+__global__ void k_totals(const int* __restrict__ gate, const long long* __restrict__ tm,
+                         long long* __restrict__ totals) {
+    if (gated_off(gate)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        totals[0] = tm[0];
+        totals[1] = tm[1] > totals[1] ? tm[1] : totals[1];
+        totals[2] = tm[0] > totals[2] ? tm[0] : totals[2];
+        totals[3] += 1;
+    }
+}
+
+struct BuildOut {
+    int* perm;
+    double* pos4;
+    long long* row_off;
+    int* nbr;
+    int* rev;
+    long long cap;
+};
+
+// The whole build.  host_totals != nullptr: synchronous (the host reads the
+// pair count and the longest row and decides on the second pass); otherwise
+// every launch is enqueued unconditionally, behind `gate`, and the totals go
+// to dev_totals.
+int build_impl(int n, const double* pos, int pos_stride, const int* types, int n_types, const double* box,
+               double rc, const BuildOut& o, void* ws, size_t ws_bytes, const int* gate,
+               long long* host_totals, long long* dev_totals, hipStream_t stream) {
     WsLayout L = ws_layout(n, n_types);
-    PT_REQUIRE(ws_bytes >= L.total, "pt_nbr_build: workspace too small (%zu < %zu)", ws_bytes,
-               L.total);
+    PT_REQUIRE(ws_bytes >= L.total, "pt_nbr_build: workspace too small (%zu < %zu)", ws_bytes, L.total);
     char* w = (char*)ws;
     int* keys_a = (int*)(w + L.keys_a);
     int* keys_b = (int*)(w + L.keys_b);
@@ -839,6 +884,7 @@ extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int*
     float4* pos4f = (float4*)(w + L.pos4f);
     unsigned long long* bmax = (unsigned long long*)(misc + 8);   // misc[0..5]: bbox, then total / max row
     const double thr = dist2_threshold(rc);
+    const bool sync = host_totals != nullptr;
 
     Grid g{};
     BoxArg ba{};
@@ -854,6 +900,8 @@ extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int*
             ba.half[d] = 0.5 * box[d];
         }
     } else {
+        // the grid of a non-periodic structure follows its bounding box, read back here
+        PT_REQUIRE(sync, "pt_nbr_build_async: periodic boxes only");
         hipLaunchKernelGGL(k_bbox, dim3(1), dim3(256), 0, stream, n, pos, pos_stride, misc);
         if (int rcode = launch_check("k_bbox")) return rcode;
         double bb[6];
@@ -884,7 +932,7 @@ extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int*
 
     // bin + sort
     const int tb = 256;
-    hipLaunchKernelGGL(k_bin, dim3((n + tb - 1) / tb), dim3(tb), 0, stream, n, pos, pos_stride, types,
+    hipLaunchKernelGGL(k_bin, dim3((n + tb - 1) / tb), dim3(tb), 0, stream, gate, n, pos, pos_stride, types,
                        g, keys_a, vals_a, bmax);
     if (int rcode = launch_check("k_bin")) return rcode;
     int bits = 1;
@@ -892,72 +940,126 @@ extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int*
     const int nblocks = (n + kTile - 1) / kTile;
     int* kin = keys_a; int* kout = keys_b; int* vin = vals_a; int* vout = vals_b;
     for (int shift = 0; shift < bits; shift += 8) {
-        hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(kSortThreads), 0, stream, n, kin,
+        hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(kSortThreads), 0, stream, gate, n, kin,
                            shift, nblocks, ghist);
-        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, stream, 256 * nblocks, ghist);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kSortThreads), 0, stream, n, kin,
+        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, stream, gate, 256 * nblocks, ghist);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kSortThreads), 0, stream, gate, n, kin,
                            vin, shift, nblocks, ghist, kout, vout);
         if (int rcode = launch_check("radix sort")) return rcode;
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
-    // kin/vin now hold the sorted keys / permutation
-    PT_CHECK_HIP(hipMemcpyAsync(perm, vin, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, stream));
-    hipLaunchKernelGGL(k_cell_start, dim3((nkeys + 1 + tb - 1) / tb), dim3(tb), 0, stream, nkeys, n,
+    // kin/vin now hold the sorted keys / permutation (copied to perm by k_gather_pos4)
+    hipLaunchKernelGGL(k_cell_start, dim3((nkeys + 1 + tb - 1) / tb), dim3(tb), 0, stream, gate, nkeys, n,
                        kin, cell_start);
-    hipLaunchKernelGGL(k_gather_pos4, dim3((n + tb - 1) / tb), dim3(tb), 0, stream, n, vin, pos,
-                       pos_stride, types, pos4, pos4f, bmax);
+    hipLaunchKernelGGL(k_gather_pos4, dim3((n + tb - 1) / tb), dim3(tb), 0, stream, gate, n, vin, pos,
+                       pos_stride, types, o.pos4, pos4f, bmax, o.perm);
     if (int rcode = launch_check("cell start / gather")) return rcode;
 
     // count + stage, scan, then compact (or a second pass for rows > kStageW)
     const int nwb = (n + 3) / 4;
-    hipLaunchKernelGGL(k_nbr_cell<PASS_STAGE>, dim3(g.ncell), dim3(kCellThreads), 0, stream, pos4,
-                       (const float4*)pos4f, (const unsigned long long*)bmax, cell_start,
-                       g, ba, n_types, thr, counts, (const long long*)nullptr, stage, heads);
+    hipLaunchKernelGGL(k_nbr_cell<PASS_STAGE>, dim3(g.ncell), dim3(kCellThreads), 0, stream, gate,
+                       (const long long*)nullptr, o.cap, o.pos4, (const float4*)pos4f,
+                       (const unsigned long long*)bmax, cell_start, g, ba, n_types, thr, counts,
+                       (const long long*)nullptr, stage, heads);
     if (int rcode = launch_check("k_nbr_cell<stage>")) return rcode;
     const int nsb = (n + kScanBlock - 1) / kScanBlock;
     long long* tot_max = (long long*)misc;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(256), 0, stream, n, counts, psum, pmax);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, nsb, psum, pmax, tot_max);
-    hipLaunchKernelGGL(k_scan_down, dim3(nsb), dim3(256), 0, stream, n, counts, psum, tot_max,
-                       row_off);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(256), 0, stream, gate, n, counts, psum, pmax);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, gate, nsb, psum, pmax, tot_max);
+    hipLaunchKernelGGL(k_scan_down, dim3(nsb), dim3(256), 0, stream, gate, n, counts, psum, tot_max, o.row_off);
     if (int rcode = launch_check("row offsets")) return rcode;
-    // The totals go to pinned host memory behind an event, and the compaction
-    // is queued before the host waits for them (it checks them itself), so the
-    // GPU does not idle through the round trip.
-    thread_local long long* host_tm = nullptr;
-    if (!host_tm) PT_CHECK_HIP(hipHostMalloc((void**)&host_tm, 2 * sizeof(long long), hipHostMallocDefault));
-    hipEvent_t ev;
-    PT_CHECK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    PT_CHECK_HIP(hipMemcpyAsync(host_tm, tot_max, 2 * sizeof(long long), hipMemcpyDeviceToHost, stream));
-    PT_CHECK_HIP(hipEventRecord(ev, stream));
-    if (nbr) {
-        hipLaunchKernelGGL(k_compact_rev, dim3(nwb), dim3(256), 0, stream, n, (const long long*)row_off,
-                           (const int*)counts, (const int*)stage, (const int*)heads, nbr, rev,
-                           (const long long*)tot_max, cap);
+    // the compaction checks the totals itself, so it is queued before the host
+    // reads them (the GPU does not idle through the round trip)
+    if (o.nbr) {
+        hipLaunchKernelGGL(k_compact_rev, dim3(nwb), dim3(256), 0, stream, gate, n, (const long long*)o.row_off,
+                           (const int*)counts, (const int*)stage, (const int*)heads, o.nbr, o.rev,
+                           (const long long*)tot_max, o.cap);
+        if (int rcode = launch_check("k_compact_rev")) return rcode;
     }
-    const int lrc = launch_check("k_compact_rev");
-    const hipError_t werr = hipEventSynchronize(ev);
-    (void)hipEventDestroy(ev);
-    PT_CHECK_HIP(werr);
-    if (lrc) return lrc;
-    *n_pairs = host_tm[0];
-    *max_row = (int)host_tm[1];
-    if (host_tm[0] > cap) {
-        set_error("pt_nbr_build: capacity %lld < %lld pairs", cap, host_tm[0]);
+    if (!sync) {
+        // rows longer than the stage: the second pass decides on the device whether it runs
+        if (o.nbr) {
+            hipLaunchKernelGGL(k_nbr_cell<PASS_FILL>, dim3(g.ncell), dim3(kCellThreads), 0, stream, gate,
+                               (const long long*)tot_max, o.cap, o.pos4, (const float4*)pos4f,
+                               (const unsigned long long*)bmax, cell_start, g, ba, n_types, thr, counts,
+                               (const long long*)o.row_off, o.nbr, (int*)nullptr);
+            if (o.rev)
+                hipLaunchKernelGGL(k_rev, dim3(nwb), dim3(256), 0, stream, gate, (const long long*)tot_max, o.cap,
+                                   n, (const long long*)o.row_off, (const int*)o.nbr, o.rev);
+        }
+        hipLaunchKernelGGL(k_totals, dim3(1), dim3(64), 0, stream, gate, (const long long*)tot_max, dev_totals);
+        return launch_check("gated build tail");
+    }
+    // synchronous: the totals are read through the caller's stack (a pageable
+    // copy, ordered on the stream) -- no pinned buffer, no event
+    long long tm[2] = {0, 0};
+    PT_CHECK_HIP(hipMemcpyAsync(tm, tot_max, sizeof(tm), hipMemcpyDeviceToHost, stream));
+    PT_CHECK_HIP(hipStreamSynchronize(stream));
+    host_totals[0] = tm[0];
+    host_totals[1] = tm[1];
+    if (tm[0] > o.cap) {
+        set_error("pt_nbr_build: capacity %lld < %lld pairs", o.cap, tm[0]);
         return PT_ERR_CAPACITY;
     }
-    PT_REQUIRE(nbr != nullptr || host_tm[0] == 0, "pt_nbr_build: nbr is null");
-    if (host_tm[1] <= kStageW) return PT_OK;
-    hipLaunchKernelGGL(k_nbr_cell<PASS_FILL>, dim3(g.ncell), dim3(kCellThreads), 0, stream, pos4,
-                       (const float4*)pos4f, (const unsigned long long*)bmax, cell_start,
-                       g, ba, n_types, thr, counts, row_off, nbr, (int*)nullptr);
+    PT_REQUIRE(o.nbr != nullptr || tm[0] == 0, "pt_nbr_build: nbr is null");
+    if (tm[1] <= kStageW) return PT_OK;
+    hipLaunchKernelGGL(k_nbr_cell<PASS_FILL>, dim3(g.ncell), dim3(kCellThreads), 0, stream, gate,
+                       (const long long*)nullptr, o.cap, o.pos4, (const float4*)pos4f,
+                       (const unsigned long long*)bmax, cell_start, g, ba, n_types, thr, counts,
+                       (const long long*)o.row_off, o.nbr, (int*)nullptr);
     if (int rcode = launch_check("k_nbr_cell<fill>")) return rcode;
-    if (rev) {
-        hipLaunchKernelGGL(k_rev, dim3(nwb), dim3(256), 0, stream, n, row_off, nbr, rev);
+    if (o.rev) {
+        hipLaunchKernelGGL(k_rev, dim3(nwb), dim3(256), 0, stream, gate, (const long long*)nullptr, o.cap, n,
+                           (const long long*)o.row_off, (const int*)o.nbr, o.rev);
         if (int rcode = launch_check("k_rev")) return rcode;
     }
     return PT_OK;
+}
+
+}  // namespace
+}  // namespace pt
+
+extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int* types, int n_types,
+                            const double* box, double rc, int* perm, double* pos4,
+                            long long* row_off, int* nbr, int* rev, long long cap,
+                            long long* n_pairs, int* max_row, void* ws, size_t ws_bytes,
+                            pt_stream_t stream_) {
+    PT_REQUIRE(n >= 1, "pt_nbr_build: n must be >= 1 (got %d)", n);
+    PT_REQUIRE(n_types >= 1 && n_types <= kMaxTypes, "pt_nbr_build: n_types must be in [1, %d]",
+               kMaxTypes);
+    PT_REQUIRE(rc > 0.0 && std::isfinite(rc), "pt_nbr_build: rc must be finite and > 0");
+    PT_REQUIRE(pos && types && perm && pos4 && row_off && n_pairs && max_row && ws,
+               "pt_nbr_build: null pointer argument");
+    PT_REQUIRE(pos_stride >= 3, "pt_nbr_build: pos_stride must be >= 3");
+    PT_REQUIRE((const void*)pos != (const void*)pos4, "pt_nbr_build: pos and pos4 must not alias");
+    BuildOut o{perm, pos4, row_off, nbr, rev, cap};
+    long long tm[2] = {0, 0};
+    const int rc_ = build_impl(n, pos, pos_stride, types, n_types, box, rc, o, ws, ws_bytes, nullptr, tm,
+                               nullptr, (hipStream_t)stream_);
+    if (rc_ == PT_OK || rc_ == PT_ERR_CAPACITY) {
+        *n_pairs = tm[0];
+        *max_row = (int)tm[1];
+    }
+    return rc_;
+}
+
+extern "C" int pt_nbr_build_async(int n, const double* pos, int pos_stride, const int* types, int n_types,
+                                  const double* box, double rc, int* perm, double* pos4, long long* row_off,
+                                  int* nbr, int* rev, long long cap, const int* gate, long long* totals,
+                                  void* ws, size_t ws_bytes, pt_stream_t stream_) {
+    PT_REQUIRE(n >= 1, "pt_nbr_build_async: n must be >= 1 (got %d)", n);
+    PT_REQUIRE(n_types >= 1 && n_types <= kMaxTypes, "pt_nbr_build_async: n_types must be in [1, %d]",
+               kMaxTypes);
+    PT_REQUIRE(rc > 0.0 && std::isfinite(rc), "pt_nbr_build_async: rc must be finite and > 0");
+    PT_REQUIRE(box != nullptr, "pt_nbr_build_async: periodic boxes only");
+    PT_REQUIRE(pos && types && perm && pos4 && row_off && nbr && totals && ws,
+               "pt_nbr_build_async: null pointer argument");
+    PT_REQUIRE(pos_stride >= 3, "pt_nbr_build_async: pos_stride must be >= 3");
+    PT_REQUIRE((const void*)pos != (const void*)pos4, "pt_nbr_build_async: pos and pos4 must not alias");
+    BuildOut o{perm, pos4, row_off, nbr, rev, cap};
+    return build_impl(n, pos, pos_stride, types, n_types, box, rc, o, ws, ws_bytes, gate, nullptr, totals,
+                      (hipStream_t)stream_);
 }
 
 extern "C" int pt_nbr_mark_rows(int n_sel, const int* sel, const long long* row_off, const int* nbr,

@@ -248,13 +248,72 @@ class ScalerWarnings(NamedTuple):
 _SCALE_TYPES = ("center", "scale", "scale_center", "scale_center_sigma")
 
 
+def _on_device(data) -> bool:
+    return hasattr(data, "is_cuda") and bool(data.is_cuda)
+
+
+def _device_rows(data):
+    """(tensor, rows, cols, ld, precision) of a 1-D/2-D FP64/FP32 device tensor."""
+    import torch
+
+    x = data if data.dim() == 2 else data.reshape(1, -1)
+    if x.dtype not in (torch.float64, torch.float32):
+        x = x.to(torch.float64)
+    if x.stride(1) != 1:
+        x = x.contiguous()
+    prec = _lib.PT_F64 if x.dtype == torch.float64 else _lib.PT_F32
+    return x, int(x.shape[0]), int(x.shape[1]), int(x.stride(0)), prec
+
+
+def params_to_device(params: "ScalerParams", device):
+    """(4, d) FP64 device tensor mean | sigma | minval | maxval (the layout of pt_scaler_*)."""
+    import torch
+
+    return torch.as_tensor(np.stack([np.asarray(params.mean, np.float64), np.asarray(params.sigma, np.float64),
+                                     np.asarray(params.minval, np.float64), np.asarray(params.maxval, np.float64)]),
+                           dtype=torch.float64, device=device).contiguous()
+
+
+def _device_partial_fit(data, params: Optional["ScalerParams"]) -> "ScalerParams":
+    """fit (params None) or partial_fit on device rows with pt_scaler_partial_fit."""
+    import torch
+
+    x, n, d, ld, prec = _device_rows(data)
+    lib = _lib.load()
+    if params is None:
+        dev_p = torch.zeros((4, d), dtype=torch.float64, device=x.device)
+        m = 0
+    else:
+        if int(params.dimension) != d:
+            raise ValueError(f"scaler dimension {params.dimension} != data width {d}")
+        dev_p = params_to_device(params, x.device)
+        m = int(params.nsamples)
+    ws = torch.empty(lib.pt_scaler_workspace_bytes(n, d), dtype=torch.uint8, device=x.device)
+    _lib.check(lib.pt_scaler_partial_fit(n, d, x.data_ptr(), ld, prec, float(m), dev_p.data_ptr(), ws.data_ptr(),
+                                         ws.numel(), _lib.stream_ptr()), "pt_scaler_partial_fit")
+    h = dev_p.cpu().numpy()
+    return ScalerParams(d, m + n, h[0].copy(), h[1].copy(), h[2].copy(), h[3].copy())
+
+
+def extrapolation_counts(data, params_dev, counters, margin: float = 0.0) -> None:
+    """Accumulate the out-of-range rows/values of device rows ``data`` into the
+    device counters (3 x uint64, pt_scaler_extrapolation); no synchronisation."""
+    x, n, d, ld, prec = _device_rows(data)
+    _lib.check(_lib.load().pt_scaler_extrapolation(n, d, x.data_ptr(), ld, prec, params_dev.data_ptr(),
+                                                   float(margin), counters.data_ptr(), _lib.stream_ptr()),
+               "pt_scaler_extrapolation")
+
+
 class DescriptorScaler:
     """Affine descriptor scaling (scaler.py:43-203, transforms :206-246).
 
     Every scaler kind is the affine map x' = (x - sub) * mul + add; the MLP
     kernel applies it on the fly, so only (sub, mul, add) reach the device.
-    Statistics (fit / partial_fit, scaler.py:249-283) are kept on the host in
-    FP64; they come from descriptors computed on the GPU.
+    Statistics (fit / partial_fit, scaler.py:249-283) and the extrapolation
+    check (scaler.py:112-142, 286-295) of device descriptor rows run on the
+    GPU (pt_scaler_partial_fit, pt_scaler_extrapolation); host arrays are
+    reduced with numpy in FP64.  The parameters themselves are small host
+    arrays.
     """
 
     def __init__(self, scale_type: str = "center", scale_min: float = 0.0, scale_max: float = 1.0):
@@ -278,11 +337,15 @@ class DescriptorScaler:
 
     @classmethod
     def fit(cls, data) -> ScalerParams:
+        if _on_device(data):
+            return _device_partial_fit(data, None)
         x = cls._np(data)
         return ScalerParams(x.shape[1], x.shape[0], x.mean(0), x.std(0), x.min(0), x.max(0))
 
     @classmethod
     def partial_fit(cls, params: ScalerParams, data) -> ScalerParams:
+        if _on_device(data):
+            return _device_partial_fit(data, params)
         x = cls._np(data)
         m, n = params.nsamples, x.shape[0]
         new_mean, new_sigma = x.mean(0), x.std(0)
@@ -321,8 +384,15 @@ class DescriptorScaler:
         """Counts one warning per call with any value outside [min, max] (scaler.py:112-142,286-295)."""
         if warnings.max_number_of_warnings < 0:
             return warnings
-        x = cls._np(data)
-        out = bool(np.any((x > params.maxval[None, :]) | (x < params.minval[None, :])))
+        if _on_device(data):
+            import torch
+
+            counters = torch.zeros(3, dtype=torch.int64, device=data.device)
+            extrapolation_counts(data, params_to_device(params, data.device), counters)
+            out = int(counters[0].item()) > 0
+        else:
+            x = cls._np(data)
+            out = bool(np.any((x > params.maxval[None, :]) | (x < params.minval[None, :])))
         new = ScalerWarnings(warnings.number_of_warnings + int(out), warnings.max_number_of_warnings)
         if new.number_of_warnings >= new.max_number_of_warnings:
             import warnings as _w

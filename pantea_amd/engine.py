@@ -99,6 +99,9 @@ class NeighborList:
                 continue
             _lib.check(rc, "pt_nbr_build")
             break
+        else:
+            raise _lib.HipLibraryError(f"pt_nbr_build: {n_pairs.value} pairs still exceed the capacity "
+                                       f"{self.cap} after three attempts")
         self.n_pairs = int(n_pairs.value)
         self.max_row = int(max_row.value)
         self.has_rev = with_rev

@@ -18,9 +18,12 @@ and ``"full"`` (-grad E).
 
 Neighbour lists are built with cutoff rc + skin and rebuilt as soon as the
 two largest atom displacements since the last build sum to more than the
-skin (checked on the device every step: no pair can have moved by more);
-the descriptor kernels re-apply the exact mask
-(r <= rc, r > 0), so every step sees exactly the reference's neighbour set.
+skin (no pair can have moved by more); the descriptor kernels re-apply the
+exact mask (r <= rc, r > 0), so every step sees exactly the reference's
+neighbour set.  In a periodic box that decision is taken on the device
+(pt_skin_check) and the rebuild runs behind its flag (pt_nbr_build_async,
+pt_permute_rows_gated), so a step never waits for the GPU; the list's
+totals are verified whenever the host reads the state.
 """
 
 from __future__ import annotations
@@ -148,9 +151,24 @@ class _DeviceMD:
 
     ``extra`` potentials (e.g. a surrogate and its judge) share the state and
     the neighbour list, whose cutoff is the largest of all plus the skin.
+
+    Periodic boxes run the step without any host read ("gated" step): the
+    skin check sets a device flag, and the rebuild (pt_nbr_build_async) and
+    the permutation of the state into the new order are launched every step
+    behind that flag, so the host never waits for the GPU.  The list's
+    totals stay on the device; they are checked when the host synchronises
+    anyway (every ``check_every`` steps of run(), a judge, an energy read).
+    The descriptor kernels are launched with a row bound (the longest row of
+    the last synchronous build plus a margin) and the list arrays are sized
+    with a margin over its pair count: a step that exceeded either is found
+    at the next check, and run() then redoes the steps since its checkpoint
+    with larger bounds.  Structures without a box use the synchronous step.
     """
 
-    def __init__(self, system: "System", skin: float, extra=()):
+    ROW_MARGIN = 16          # row bound = longest row + this
+    PAIR_MARGIN = 1.3        # list arrays = pairs * this
+
+    def __init__(self, system: "System", skin: float, extra=(), check_every: int = 50, sync_free: bool = True):
         import torch
 
         pot = system.potential
@@ -160,21 +178,32 @@ class _DeviceMD:
         self.emap = s.element_map
         self.box = s.box_lengths
         self.skin = float(skin)
+        self.gated = self.box is not None and sync_free
+        self.check_every = int(check_every)
         n = self.n
         self.fields = {(id(pot), system.precision): _ForceField(pot, system.precision, self.emap, n, self.dev)}
         for p, prec in extra:
             self.fields.setdefault((id(p), prec), _ForceField(p, prec, self.emap, n, self.dev))
         self.ff = self.fields[(id(pot), system.precision)]
         self.rc = max(f.rc for f in self.fields.values())
-        self.nls = [NeighborList(self.rc + self.skin, self.emap.n_types),
-                    NeighborList(self.rc + self.skin, self.emap.n_types)]
-        self.cur = 0
         self.type_counts = s.type_counts()
+        self.timers = None  # {phase: [(start_event, end_event), ...]} when profiling
+        self._builds = 0
+        self.redos = 0        # stretches redone after a list overflow
+        self._since = 0       # gated steps since the last verified checkpoint
+        self._ck = None
+        self._args = None
+        self.flag = torch.ones(1, dtype=torch.int32, device=self.dev)
+        self.totals = torch.zeros(4, dtype=torch.int64, device=self.dev)
+        self.disp = torch.zeros(514, dtype=torch.float64, device=self.dev)
+        self.reducer = Reducer(n, self.dev)
         # state in input order -> internal order of the first build
-        nl = self.nls[0].build(s.positions, s.atom_types, self.box, self.type_counts, with_rev=True)
+        self._nl = NeighborList(self.rc + self.skin, self.emap.n_types)
+        nl = self._nl.build(s.positions, s.atom_types, self.box, self.type_counts, with_rev=True)
+        self._builds += 1
         perm = nl.perm
         self.orig = perm.clone()                                     # internal -> input index
-        self.types = gather_rows(perm, s.atom_types.to(torch.int32).reshape(-1, 1)).reshape(-1)
+        self.types = gather_rows(perm, s.atom_types.to(torch.int32).reshape(-1, 1)).reshape(-1).contiguous()
         self.vel = gather_rows(perm, system.velocities.contiguous())
         mass = system.masses.reshape(-1, 1).contiguous()
         self.mass = gather_rows(perm, mass).reshape(-1).contiguous()
@@ -182,10 +211,28 @@ class _DeviceMD:
         self.ref_pos = nl.pos4.clone()
         self.force = torch.zeros((n, 3), dtype=torch.float64, device=self.dev)
         self.force_new = torch.zeros_like(self.force)
-        self.disp = torch.zeros(514, dtype=torch.float64, device=self.dev)
-        self.reducer = Reducer(n, self.dev)
-        self.n_builds = 1
-        self.timers = None  # {phase: [(start_event, end_event), ...]} when profiling
+        self._size_bounds()
+
+    # ---- list bounds of the gated step ------------------------------------------
+    def _size_bounds(self, grow: bool = False) -> None:
+        """Row bound and list capacity from the last synchronous build (grown after an overflow)."""
+        import torch
+
+        nl = self._nl
+        self.pos_in = torch.empty_like(nl.pos4)
+        self.tmp3 = torch.empty((self.n, 3), dtype=torch.float64, device=self.dev)
+        self.tmp1 = torch.empty(self.n, dtype=torch.float64, device=self.dev)
+        self.tmpi = torch.empty(self.n, dtype=torch.int32, device=self.dev)
+        f = 2.0 if grow else 1.0
+        self.row_bound = int(f * nl.max_row) + self.ROW_MARGIN
+        cap = int(f * self.PAIR_MARGIN * nl.n_pairs) + 1024
+        if cap > nl.cap:
+            nl.cap = cap
+            nl.nbr = torch.empty(cap, dtype=torch.int32, device=self.dev)
+            nl.rev = torch.empty(cap, dtype=torch.int32, device=self.dev)
+            # the list content was lost with the arrays: build it again
+            self._rebuild_sync(nl.pos4.clone())
+        self.totals.zero_()
 
     def add_field(self, pot, precision: str) -> _ForceField:
         """Evaluate ``pot`` at ``precision`` on this state too (grows the list cutoff if needed)."""
@@ -195,11 +242,10 @@ class _DeviceMD:
             rc = max(f.rc for f in self.fields.values())
             if rc > self.rc:
                 self.rc = rc
-                pos = self.nl.pos4.clone()
-                self.nls = [NeighborList(rc + self.skin, self.emap.n_types),
-                            NeighborList(rc + self.skin, self.emap.n_types)]
-                self.cur = 0
-                self._rebuild(pos)
+                pos = self._nl.pos4.clone()
+                self._nl = NeighborList(rc + self.skin, self.emap.n_types)
+                self._rebuild_sync(pos)
+                self._size_bounds()
         return self.fields[key]
 
     # the driving potential's buffers (kept for callers that read them)
@@ -236,32 +282,84 @@ class _DeviceMD:
 
     @property
     def nl(self) -> NeighborList:
-        return self.nls[self.cur]
+        return self._nl
+
+    @property
+    def n_builds(self) -> int:
+        """Neighbour builds so far (reads the device counter of the gated steps: a sync)."""
+        return self._builds + (int(self.totals[3].item()) if self.gated else 0)
 
     # ---- neighbour maintenance ---------------------------------------------
-    def _rebuild(self, positions=None) -> None:
-        """New list (and internal order) from ``positions`` (current order; default: current positions)."""
-        old = self.nls[self.cur]
-        new = self.nls[1 - self.cur]
-        new.build(old.pos4 if positions is None else positions, self.types, self.box, self.type_counts,
-                  with_rev=True)
-        p = new.perm
+    def _permute_state(self, p) -> None:
         self.orig = gather_rows(p, self.orig.reshape(-1, 1)).reshape(-1).contiguous()
         self.types = gather_rows(p, self.types.reshape(-1, 1)).reshape(-1).contiguous()
         self.vel = gather_rows(p, self.vel)
         self.force = gather_rows(p, self.force)
         self.mass = gather_rows(p, self.mass.reshape(-1, 1)).reshape(-1).contiguous()
         self.inv_mass = gather_rows(p, self.inv_mass.reshape(-1, 1)).reshape(-1).contiguous()
-        self.ref_pos.copy_(new.pos4)
-        self.cur = 1 - self.cur
-        self.n_builds += 1
+
+    def _rebuild_sync(self, positions=None) -> None:
+        """New list and internal order from ``positions`` (current internal order, n x 3 or 4),
+        read back synchronously."""
+        nl = self._nl
+        src = nl.pos4.clone() if positions is None else positions
+        nl.build(src, self.types, self.box, self.type_counts, with_rev=True)
+        self._permute_state(nl.perm)
+        self.ref_pos.copy_(nl.pos4)
+        self._builds += 1
+        self.row_bound = max(getattr(self, "row_bound", 0), nl.max_row)
+
+    def _rebuild_gated(self) -> None:
+        """Skin check, then (behind its device flag) rebuild and reorder the state: no host read."""
+        lib = _lib.load()
+        nl = self._nl
+        st = _lib.stream_ptr()
+        box = host_box(self.box)
+        box_p = ctypes.cast(box, ctypes.c_void_p)
+        g = self.flag.data_ptr()
+        _lib.check(lib.pt_skin_check(self.n, nl.pos4.data_ptr(), self.ref_pos.data_ptr(), box_p, self.skin,
+                                     self.disp.data_ptr(), g, st), "pt_skin_check")
+        _lib.check(lib.pt_copy_gated(4 * self.n, 8, nl.pos4.data_ptr(), self.pos_in.data_ptr(), g, st),
+                   "pt_copy_gated")
+        _lib.check(lib.pt_nbr_build_async(self.n, self.pos_in.data_ptr(), 4, self.types.data_ptr(), nl.n_types,
+                                          box_p, nl.r_cutoff, nl.perm.data_ptr(), nl.pos4.data_ptr(),
+                                          nl.row_off.data_ptr(), nl.nbr.data_ptr(), nl.rev.data_ptr(), nl.cap, g,
+                                          self.totals.data_ptr(), nl.ws.data_ptr(), nl.ws.numel(), st),
+                   "pt_nbr_build_async")
+        perm = nl.perm.data_ptr()
+        for buf, width, wb, tmp in ((self.vel, 3, 8, self.tmp3), (self.force, 3, 8, self.tmp3),
+                                    (self.mass, 1, 8, self.tmp1), (self.inv_mass, 1, 8, self.tmp1),
+                                    (self.orig, 1, 4, self.tmpi), (self.types, 1, 4, self.tmpi)):
+            _lib.check(lib.pt_permute_rows_gated(self.n, width, wb, perm, buf.data_ptr(), tmp.data_ptr(), g, st),
+                       "pt_permute_rows_gated")
+        _lib.check(lib.pt_copy_gated(4 * self.n, 8, nl.pos4.data_ptr(), self.ref_pos.data_ptr(), g, st),
+                   "pt_copy_gated")
+        # the descriptor kernels stage rows up to the bound; buffers are sized for the capacity
+        nl.max_row = self.row_bound
+        nl.n_pairs = nl.cap
+
+    def lists_ok(self) -> bool:
+        """Whether every gated build since the last check fitted the row bound and the
+        list arrays (a host read: call where the host synchronises anyway)."""
+        if not self.gated:
+            return True
+        t = self.totals.cpu().numpy()
+        self._builds += int(t[3])
+        ok = t[1] <= self.row_bound and t[2] <= self._nl.cap
+        self.totals[1:].zero_()
+        return bool(ok)
+
+    def grow_lists(self) -> None:
+        """Larger row bound and list arrays after an overflow (synchronous rebuild)."""
+        self._rebuild_sync()
+        self._size_bounds(grow=True)
 
     def _stale(self) -> bool:
         if self.skin <= 0.0:
             return True
         lib = _lib.load()
         box = host_box(self.box)
-        _lib.check(lib.pt_max_displacement(self.n, self.nl.pos4.data_ptr(), self.ref_pos.data_ptr(),
+        _lib.check(lib.pt_max_displacement(self.n, self._nl.pos4.data_ptr(), self.ref_pos.data_ptr(),
                                            ctypes.cast(box, ctypes.c_void_p) if box is not None else None,
                                            self.disp.data_ptr(), _lib.stream_ptr()), "pt_max_displacement")
         d1, d2 = self.disp[:2].tolist()   # the two largest displacements
@@ -269,7 +367,7 @@ class _DeviceMD:
 
     # ---- state snapshots (rewind) -------------------------------------------------
     def save_state(self) -> dict:
-        return {"pos": self.nl.pos4.clone(), "vel": self.vel.clone(), "force": self.force.clone(),
+        return {"pos": self._nl.pos4.clone(), "vel": self.vel.clone(), "force": self.force.clone(),
                 "orig": self.orig.clone(), "types": self.types.clone(), "mass": self.mass.clone(),
                 "inv_mass": self.inv_mass.clone()}
 
@@ -280,13 +378,13 @@ class _DeviceMD:
         self.force = st["force"].clone()
         self.mass = st["mass"].clone()
         self.inv_mass = st["inv_mass"].clone()
-        self._rebuild(st["pos"])
+        self._rebuild_sync(st["pos"].clone())
 
     # ---- forces --------------------------------------------------------------
     def compute(self, out, force_mode: str, with_energy: bool = False, field: Optional[_ForceField] = None):
         f = field or self.ff
         tag = "" if len(self.fields) == 1 else f"{f.label}/"
-        return f.compute(self.nl, out, force_mode, with_energy, lambda ph, st: self._mark(tag + ph, st))
+        return f.compute(self._nl, out, force_mode, with_energy, lambda ph, st: self._mark(tag + ph, st))
 
     # ---- one step ------------------------------------------------------------
     def step(self, dt: float, integrator: str, force_mode: str, thermostat: Optional[BrendsenThermostat]):
@@ -294,13 +392,17 @@ class _DeviceMD:
         inv_m = self.inv_mass.data_ptr() if integrator == "physical" else None
         box = host_box(self.box)
         box_p = ctypes.cast(box, ctypes.c_void_p) if box is not None else None
-        _lib.check(lib.pt_verlet_positions(self.n, self.nl.pos4.data_ptr(), self.vel.data_ptr(),
+        _lib.check(lib.pt_verlet_positions(self.n, self._nl.pos4.data_ptr(), self.vel.data_ptr(),
                                            self.force.data_ptr(), inv_m, dt, box_p, _lib.stream_ptr()),
                    "pt_verlet_positions")
-        if self._stale():
-            self._mark("nbr_build", True)
-            self._rebuild()
-            self._mark("nbr_build", False)
+        self._mark("nbr_build", True)
+        if self.gated:
+            self._rebuild_gated()
+            inv_m = self.inv_mass.data_ptr() if integrator == "physical" else None
+        elif self._stale():
+            self._rebuild_sync()
+            inv_m = self.inv_mass.data_ptr() if integrator == "physical" else None
+        self._mark("nbr_build", False)
         self.compute(self.force_new, force_mode)
         _lib.check(lib.pt_verlet_velocities(self.n, self.vel.data_ptr(), self.force.data_ptr(),
                                             self.force_new.data_ptr(), inv_m, dt, _lib.stream_ptr()),
@@ -312,6 +414,38 @@ class _DeviceMD:
                                                 thermostat.time_constant, thermostat.target_temperature,
                                                 KB, _lib.stream_ptr()), "pt_berendsen_rescale")
 
+    def advance(self, steps: int, dt: float, integrator: str, force_mode: str,
+                thermostat: Optional[BrendsenThermostat]) -> None:
+        """``steps`` steps.  Gated steps are verified every ``check_every`` steps (and
+        whenever the host reads the state): a stretch whose lists overflowed is redone
+        from its checkpoint with larger bounds."""
+        args = (dt, integrator, force_mode, thermostat)
+        for _ in range(int(steps)):
+            if self._args is not None and self._args != args:
+                self.verify()
+            if self.gated and self._since == 0:
+                self._ck = self.save_state()
+            self._args = args
+            self.step(*args)
+            if self.gated:
+                self._since += 1
+                if self._since >= self.check_every:
+                    self.verify()
+
+    def verify(self) -> None:
+        """Check the gated builds since the last checkpoint (a host read) and redo the
+        steps since then with larger bounds if a list overflowed."""
+        if not self.gated or self._since == 0:
+            return
+        k = self._since
+        self._since = 0
+        while not self.lists_ok():
+            self.redos += 1
+            self.restore_state(self._ck)
+            self.grow_lists()
+            for _ in range(k):
+                self.step(*self._args)
+
     # ---- views in input order ----------------------------------------------------
     def to_input(self, internal):
         import torch
@@ -320,7 +454,7 @@ class _DeviceMD:
         return scatter_rows(self.orig, internal.reshape(internal.shape[0], -1), out.reshape(out.shape[0], -1))
 
     def positions(self):
-        return self.to_input(self.nl.pos4[:, :3].contiguous())
+        return self.to_input(self._nl.pos4[:, :3].contiguous())
 
 
 class System:
@@ -331,9 +465,11 @@ class System:
     """
 
     def __init__(self, potential: NeuralNetworkPotential, structure, velocities, masses,
-                 force_mode: str = "reference", precision: str = "float64", skin: float = 0.0):
+                 force_mode: str = "reference", precision: str = "float64", skin: float = 0.0,
+                 sync_free: bool = True):
         if force_mode not in ("reference", "full"):
             raise ValueError(f"unknown force mode '{force_mode}'")
+        self.sync_free = bool(sync_free)
         self.potential = potential
         self.structure = structure
         self._velocities = velocities
@@ -347,6 +483,7 @@ class System:
     @property
     def velocities(self):
         if self._md is not None:
+            self._md.verify()
             self._velocities = self._md.to_input(self._md.vel)
         return self._velocities
 
@@ -357,16 +494,17 @@ class System:
         value = torch.as_tensor(value, dtype=torch.float64, device=self.masses.device).reshape(-1, 3).contiguous()
         self._velocities = value
         if self._md is not None:
+            self._md.verify()
             self._md.vel = gather_rows(self._md.orig, value)
 
     @classmethod
     def from_structure(cls, structure, potential, temperature: float = 300.0, seed: int = 2024,
                        force_mode: str = "reference", precision: Optional[str] = None,
-                       skin: float = 0.0) -> "System":
+                       skin: float = 0.0, sync_free: bool = True) -> "System":
         masses = structure.element_map.get_masses_from_structure(structure)
         vel = cls.generate_random_velocities(temperature, masses, seed)
         prec = precision or getattr(potential, "precision", "float64")
-        return cls(potential, structure.copy(), vel, masses, force_mode, prec, skin)
+        return cls(potential, structure.copy(), vel, masses, force_mode, prec, skin, sync_free)
 
     @staticmethod
     def generate_random_velocities(temperature: float, masses, seed: int):
@@ -384,13 +522,14 @@ class System:
     # ---- device state ------------------------------------------------------------
     def _engine(self, extra=()) -> _DeviceMD:
         if self._md is None:
-            self._md = _DeviceMD(self, self.skin, extra)
+            self._md = _DeviceMD(self, self.skin, extra, sync_free=self.sync_free)
         return self._md
 
     def _sync_from_engine(self) -> None:
         md = self._md
         if md is None:
             return
+        md.verify()
         self.structure.positions = md.positions().to(self.structure.dtype)
         self.structure.forces = md.to_input(md.force)
 
@@ -405,11 +544,13 @@ class System:
     # ---- observables (system.py:115-140) -----------------------------------------
     def get_potential_energy(self):
         md = self._engine()
+        md.verify()
         md.compute(md.force_new, self.force_mode, with_energy=True)
         return md.energy.clone()
 
     def get_kinetic_energy(self):
         md = self._engine()
+        md.verify()
         return md.reducer.kinetic(md.vel, md.mass).clone()
 
     def get_total_energy(self):
@@ -469,14 +610,13 @@ class MDSimulator:
         self.elapsed_time = 0.0
 
     def simulate_one_step(self, system: System) -> None:
-        system._engine().step(self.time_step, self.integrator, system.force_mode, self.thermostat)
+        system._engine().advance(1, self.time_step, self.integrator, system.force_mode, self.thermostat)
         self.step += 1
         self.elapsed_time += self.time_step
 
     def run(self, system: System, num_steps: int) -> None:
         md = system._engine()
-        for _ in range(num_steps):
-            md.step(self.time_step, self.integrator, system.force_mode, self.thermostat)
+        md.advance(num_steps, self.time_step, self.integrator, system.force_mode, self.thermostat)
         self.step += num_steps
         self.elapsed_time += num_steps * self.time_step
 

@@ -151,7 +151,7 @@ class StudentTrainer:
                         if e2 != el:
                             continue
                         n_sf = pot.atomic_potentials[el].model_input_size
-                        x = sn.G[b:b + cnt, :n_sf].double().cpu().numpy()
+                        x = sn.G[b:b + cnt, :n_sf]   # device rows: statistics on the GPU
                         params = DescriptorScaler.fit(x) if params is None else DescriptorScaler.partial_fit(params, x)
                 pot.scalers_params[el] = params
             pot._mlp_key = None

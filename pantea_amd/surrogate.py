@@ -56,6 +56,7 @@ class SurrogateStats:
     fallback_failures: int = 0     # of those, failures of the judged fallback
     judge_rmse: list = field(default_factory=list)   # Ha/Bohr, one per judge
     judge_driver: list = field(default_factory=list)  # "student" or "fallback", one per judge
+    list_overflows: int = 0        # judged windows redone because a gated list build overflowed
     seconds: float = 0.0
 
     @property
@@ -74,7 +75,7 @@ class SurrogateStats:
         return {"steps": self.steps, "student_steps": self.student_steps, "fallback_steps": self.fallback_steps,
                 "reference_steps": self.reference_steps,
                 "discarded_steps": self.discarded_steps, "judges": self.judges, "failures": self.failures,
-                "fallback_failures": self.fallback_failures,
+                "fallback_failures": self.fallback_failures, "list_overflows": self.list_overflows,
                 "fallback_rate": self.fallback_rate, "teacher_share": self.teacher_share,
                 "judge_rmse_mev_per_bohr_median": float(np.median(r)) if r.size else None,
                 "judge_rmse_mev_per_bohr_max": float(np.max(r)) if r.size else None,
@@ -219,6 +220,11 @@ class SurrogateMD:
         """Judge the driver at the current positions; on a pass the state becomes the rewind point."""
         st = self.stats
         rmse = self.judge()
+        # the judge synchronises: check the gated list builds of the window with it
+        if not self.md.lists_ok():
+            self.md.grow_lists()
+            st.list_overflows += 1
+            rmse = math.inf
         st.judges += 1
         st.judge_rmse.append(rmse)
         st.judge_driver.append(self.mode)
@@ -239,6 +245,21 @@ class SurrogateMD:
         st = self.stats
         target = st.steps + int(n_steps)
         t0 = time.perf_counter()
+        while True:
+            self._advance(target)
+            # steps after the last judge: their list builds are checked before they count
+            if md.lists_ok():
+                break
+            md.grow_lists()
+            st.list_overflows += 1
+            self._to_fallback()
+        torch.cuda.synchronize()
+        st.seconds += time.perf_counter() - t0
+        return st
+
+    def _advance(self, target: int) -> None:
+        md = self.md
+        st = self.stats
         while st.steps < target:
             md.step(self.dt, self.integrator, "full", self.thermostat)
             st.steps += 1
@@ -265,9 +286,6 @@ class SurrogateMD:
                 self._to_fallback(reference=True)
             elif self.fallback_left == 0:
                 self._to_student()
-        torch.cuda.synchronize()
-        st.seconds += time.perf_counter() - t0
-        return st
 
     @property
     def positions(self):

@@ -4,8 +4,9 @@ Each check is chosen so that a plausible mistake (dropped term, wrong sign or
 index, missing halving, wrong MIC) fails one of them:
   * the reference's golden vectors (test_acsf.py, test_nnp.py, test_structure.py),
   * closed forms (cutoff functions, scaler statistics),
-  * brute force (neighbour sets from periodic images; G3 by an explicit
-    ordered-pair triple loop),
+  * brute force (neighbour sets from periodic images; G3 and G9 by explicit
+    pair loops, G9 also by a three-atom closed form),
+  * every MLP activation against its closed form and finite differences,
   * finite differences (self-Jacobian, self-term forces, -grad E),
   * invariants (sum of -grad E = 0, rotation invariance).
 """
@@ -170,6 +171,44 @@ def test_mlp_gradient_fd():
     assert np.allclose(y1, (x @ W + b)[:, 0], rtol=1e-15) and np.allclose(g1, np.tile(W.T, (3, 1)))
 
 
+# closed forms of models/nn/activation.py:7-60 (n2p2 definitions): f(z)
+_ACT_CLOSED = {
+    "identity": lambda z: z,
+    "tanh": math.tanh,
+    "logistic": lambda z: 1.0 / (1.0 + math.exp(-z)),
+    "softplus": lambda z: math.log1p(math.exp(z)),
+    "relu": lambda z: max(z, 0.0),
+    "gaussian": lambda z: math.exp(-0.5 * z * z),
+    "cos": math.cos,
+    "exp": lambda z: math.exp(-z),
+    "harmonic": lambda z: z * z,
+}
+
+
+@pytest.mark.parametrize("act", sorted(_ACT_CLOSED))
+def test_activation_values_and_derivatives(act):
+    """Every activation: value against its closed form, derivative against a
+    central finite difference of that closed form (relu away from 0), then the
+    same activation inside a hidden layer through the MLP input gradient."""
+    zs = np.array([-2.3, -0.7, -0.05, 0.11, 0.9, 1.7])
+    for z in zs:
+        y, d = O._act(act, np.array([z]))
+        assert y[0] == pytest.approx(_ACT_CLOSED[act](z), rel=1e-14, abs=1e-15)
+        h = 1e-6
+        fd = (_ACT_CLOSED[act](z + h) - _ACT_CLOSED[act](z - h)) / (2 * h)
+        assert d[0] == pytest.approx(fd, rel=1e-7, abs=1e-9)
+    rng = np.random.default_rng(sum(map(ord, act)))
+    layers = [(rng.normal(size=(4, 6)) * 0.5, rng.normal(size=6) * 0.1, act),
+              (rng.normal(size=(6, 1)), rng.normal(size=1), "identity")]
+    x = rng.normal(size=(5, 4))
+    _, g = O.mlp_forward_backward(layers, x)
+    h = 1e-6
+    for k in range(4):
+        e = np.zeros(4); e[k] = h
+        fd = (O.mlp_forward_backward(layers, x + e)[0] - O.mlp_forward_backward(layers, x - e)[0]) / (2 * h)
+        assert np.allclose(g[:, k], fd, rtol=1e-6, atol=1e-8)
+
+
 # ---------------------------------------------------------------------------
 # brute force
 # ---------------------------------------------------------------------------
@@ -221,6 +260,77 @@ def test_g3_halving_matches_unordered_pairs():
                 f = lambda r: 0.5 * (math.cos(math.pi * r / rc) + 1) if r < rc else 0.0
                 acc += 2 ** (1 - 4.0) * (1 - c) ** 4 * math.exp(-0.03 * (rj * rj + rk * rk + rjk * rjk)) * f(rj) * f(rk) * f(rjk)
             assert G[i] == pytest.approx(acc, rel=1e-12, abs=1e-15)
+
+
+def _g9_brute(pos, types, box, ej, ek, eta, zeta, lam, rc, fc):
+    """G9 of every atom by an explicit loop over ORDERED pairs (j, k), k != j,
+    following angular.py:92-107 and acsf.py:294-330: no cutoff on r_jk, no
+    r_jk in the exponent, halved when the two neighbour elements coincide."""
+    n = len(types)
+    out = np.zeros(n)
+    for i in range(n):
+        nb = []
+        for j in range(n):
+            d = pos[i] - pos[j]
+            if box is not None:
+                d = d - box * np.round(d / box)
+            r = np.linalg.norm(d)
+            if 0 < r <= rc:
+                nb.append((j, d, r))
+        acc = 0.0
+        for (j, dj, rj) in nb:
+            if types[j] != ej:
+                continue
+            for (k, dk, rk) in nb:
+                if types[k] != ek or k == j:
+                    continue
+                c = dj @ dk / (rj * rk)
+                acc += 2.0 ** (1 - zeta) * (1 + lam * c) ** zeta * math.exp(-eta * (rj * rj + rk * rk)) * fc(rj) * fc(rk)
+        out[i] = acc * (0.5 if ej == ek else 1.0)
+    return out
+
+
+@pytest.mark.parametrize("ej,ek,zeta,lam", [(1, 1, 4.0, -1.0), (1, 2, 1.0, 1.0), (2, 2, 2.0, 1.0), (1, 2, 3.0, -1.0)])
+def test_g9_matches_ordered_pair_brute_force(ej, ek, zeta, lam):
+    """G9 (angular.py:92-107) against an ordered-pair loop, with a cutoff
+    larger than half the box so pairs with r_jk > rc occur (they count in G9
+    and not in G3)."""
+    pos, types, box = _small_water(4, 9.0, seed=3)
+    rc = 5.5
+    tanh_pre = O.tanh_prefactor()
+    fc = lambda r: tanh_pre * math.tanh(1 - r / rc) ** 3 if r < rc else 0.0
+    row = [9, ej, ek, 0.02, 0.0, zeta, lam, rc, TANH, 0]
+    G = O.acsf(pos, types, box, [row])[:, 0]
+    want = _g9_brute(pos, types, box, ej, ek, 0.02, zeta, lam, rc, fc)
+    assert np.abs(want).max() > 1e-3
+    assert np.allclose(G, want, rtol=1e-12, atol=1e-15)
+    # a G9 that took fc(r_jk) or r_jk^2 into account, as G3 does, would differ
+    g3 = O.acsf(pos, types, box, [[3] + row[1:]])[:, 0]
+    assert np.abs(G - g3).max() > 1e-3 * np.abs(G).max()
+
+
+def test_g9_three_atom_closed_form():
+    """Centre O at the origin, H at (a, 0, 0) and O at (0, b, 0): cos = 0 and
+    r_jk = sqrt(a^2 + b^2) > rc, so G3 = 0 (fc(r_jk) = 0) while
+    G9 = 2^(1-zeta) exp(-eta (a^2 + b^2)) fc(a) fc(b) (distinct elements: no
+    halving, one ordered pair); the same pair seen from the H centre has
+    cos = a / r_jk and only one neighbour within rc (G9 = 0)."""
+    a, b, rc, eta, zeta = 2.0, 3.0, 3.4, 0.07, 4.0
+    pos = np.array([[0.0, 0.0, 0.0], [a, 0.0, 0.0], [0.0, b, 0.0]])
+    types = np.array([2, 1, 2], dtype=np.int32)
+    fc = lambda r: 0.5 * (math.cos(math.pi * r / rc) + 1) if r < rc else 0.0
+    for lam in (1.0, -1.0):
+        G9 = O.acsf(pos, types, None, [[9, 1, 2, eta, 0.0, zeta, lam, rc, COS, 0]])[:, 0]
+        G3 = O.acsf(pos, types, None, [[3, 1, 2, eta, 0.0, zeta, lam, rc, COS, 0]])[:, 0]
+        expect = 2.0 ** (1 - zeta) * math.exp(-eta * (a * a + b * b)) * fc(a) * fc(b)
+        assert G9[0] == pytest.approx(expect, rel=1e-14)
+        assert G9[1] == 0.0 and G9[2] == 0.0 and G3[0] == 0.0
+    # inside the cutoff both see the pair; with cos = 0 the lambda sign drops out
+    rc2 = 4.0
+    fc2 = lambda r: 0.5 * (math.cos(math.pi * r / rc2) + 1)
+    rjk = math.hypot(a, b)
+    G3 = O.acsf(pos, types, None, [[3, 1, 2, eta, 0.0, 1.0, -1.0, rc2, COS, 0]])[:, 0]
+    assert G3[0] == pytest.approx(math.exp(-eta * (a * a + b * b + rjk * rjk)) * fc2(a) * fc2(b) * fc2(rjk), rel=1e-14)
 
 
 # ---------------------------------------------------------------------------
@@ -295,3 +405,38 @@ def test_descriptor_rotation_invariance():
     G0 = O.acsf(pos, types, None, rows)
     G1 = O.acsf(pos @ R.T + 3.0, types, None, rows)
     assert np.allclose(G0, G1, rtol=1e-12, atol=1e-14)
+
+
+def test_reference_algorithm_matches_oracle():
+    """The dense reference algorithm (timed as the CPU baseline) gives the oracle's
+    descriptors and, with weights w, the self-term gradient sum_s w_s dG_is/dr_i."""
+    pos, types, box = _small_water(6, 12.0, seed=4)
+    rows = _teacher_rows(rc=5.0)
+    centres = np.arange(0, len(types), 2).astype(np.int32)
+    G, _ = O.reference_algorithm(pos, types, box, rows, centres)
+    G0 = O.acsf(pos, types, box, rows, center_pos=pos[centres], self_idx=centres)
+    np.testing.assert_allclose(G, G0, rtol=1e-12, atol=1e-14)
+    rng = np.random.default_rng(2)
+    w = rng.normal(size=G.shape)
+    _, grad = O.reference_algorithm(pos, types, box, rows, centres, w)
+    J = O.acsf_selfjac(pos, types, box, rows, center_pos=pos[centres], self_idx=centres)
+    np.testing.assert_allclose(grad, np.einsum("cs,csd->cd", w, J), rtol=1e-10, atol=1e-12)
+
+
+def test_cell_list_rows_and_centres_match_dense_oracle():
+    """The cell-list CPU baseline: same rows as the dense mask, same descriptors and
+    self gradients as or_acsf / the self-Jacobian."""
+    pos, types, box = _small_water(40, 19.0, seed=8)
+    rc = 6.0
+    counts, idx = O.neighbors(pos, box, rc)
+    row_off, idx2 = O.cell_neighbors(pos, box, rc)
+    assert np.array_equal(np.diff(row_off), counts) and np.array_equal(idx2, idx)
+    rows = _teacher_rows(rc=rc)
+    centres = np.arange(1, len(types), 3).astype(np.int32)
+    rng = np.random.default_rng(4)
+    w = rng.normal(size=(centres.size, rows.shape[0]))
+    G, g = O.celllist_centres(pos, types, box, row_off, idx2, rows, centres, w)
+    np.testing.assert_allclose(G, O.acsf(pos, types, box, rows, center_pos=pos[centres], self_idx=centres),
+                               rtol=1e-13, atol=1e-15)
+    J = O.acsf_selfjac(pos, types, box, rows, center_pos=pos[centres], self_idx=centres)
+    np.testing.assert_allclose(g, np.einsum("cs,csd->cd", w, J), rtol=1e-11, atol=1e-13)

@@ -332,6 +332,16 @@ size_t pt_reduce_workspace_bytes(int n);
  * potentials/nnp/energy.py:54-63. */
 int pt_reduce_sum_f64(int n, const double *x, double *out, void *ws, size_t ws_bytes,
                       pt_stream_t stream);
+/* The same sum without a workspace and without any allocation: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It has no counterpart in
+ * the reference.  The partial sums live in a 512 KiB device array of the
+ * library, one copy per GPU, held until the process ends; n is limited to
+ * 2^26 = 67,108,864 elements (PT_ERR_ARG beyond, nothing launched).  The
+ * stream must belong to the current device.  Calls that overlap on one GPU
+ * (several streams or threads) share that array and race.  Results are
+ * bitwise those of pt_reduce_sum_f64 (same kernels, same order). */
+int pt_reduce_sum_f64_noalloc(int n, const double *x, double *out, pt_stream_t stream);
 
 /* x <- wrap(x + v dt + 1/2 a dt^2), a = F (inv_mass == NULL, the reference
  * integrator of simulation/molecular_dynamics.py:16-20,65-72) or F/m.

@@ -120,6 +120,7 @@ _SIGS = {
     "pt_scaler_extrapolation": (_c_int, [_c_int, _c_int, _vp, _c_int, _c_int, _vp, _c_dbl, _vp, _vp]),
     "pt_reduce_workspace_bytes": (_c_size, [_c_int]),
     "pt_reduce_sum_f64": (_c_int, [_c_int, _vp, _vp, _vp, _c_size, _vp]),
+    "pt_reduce_sum_f64_noalloc": (_c_int, [_c_int, _vp, _vp, _vp]),
     "pt_verlet_positions": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_dbl, _vp, _vp]),
     "pt_verlet_velocities": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_dbl, _vp]),
     "pt_kinetic_energy": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_size, _vp]),

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 
 #include "common.h"
 
@@ -293,6 +294,64 @@ extern "C" int pt_reduce_sum_f64(int n, const double* x, double* out, void* ws, 
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, stream, nb, (const double*)ws,
                        out);
     return launch_check("pt_reduce_sum_f64");
+}
+
+// ---- FP64 sum without allocation ----------------------------------------------
+// The partials of pt_reduce_sum_f64_noalloc live in `scratch`, a file-scope
+// device array: every GPU the library runs on holds its own copy (512 KiB)
+// until the process ends.  Its address on a device comes from
+// hipGetSymbolAddress, taken once per device with that device current; host
+// code uses only that address (the array's name is not a device pointer on
+// the host).  k_reduce_partial writes every partial that k_reduce_final reads,
+// so nothing needs clearing between calls; calls that overlap on one GPU
+// (several streams or threads) share the array and race.
+namespace pt {
+constexpr int kNoAllocMaxN = 1 << 26;                       // 67,108,864 elements
+constexpr int kNoAllocPartials = kNoAllocMaxN / kRedTile;   // 65,536 partial sums
+}  // namespace pt
+
+static __device__ __align__(256) double scratch[pt::kNoAllocPartials];
+
+namespace pt {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on its first use there).
+int scratch_on_current_device(double** out) {
+    int dev = -1;
+    PT_CHECK_HIP(hipGetDevice(&dev));
+    PT_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        PT_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        PT_REQUIRE(p != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PT_OK;
+}
+
+}  // namespace
+}  // namespace pt
+
+extern "C" int pt_reduce_sum_f64_noalloc(int n, const double* x, double* out, pt_stream_t stream_) {
+    hipStream_t stream = (hipStream_t)stream_;
+    PT_REQUIRE(n >= 0 && out && (n == 0 || x), "pt_reduce_sum_f64_noalloc: bad arguments");
+    PT_REQUIRE(n <= pt::kNoAllocMaxN,
+               "pt_reduce_sum_f64_noalloc: n = %d exceeds the maximum of %d elements "
+               "(pt_reduce_sum_f64 takes a workspace for larger sums)", n, pt::kNoAllocMaxN);
+    double* part = nullptr;
+    if (int rc = pt::scratch_on_current_device(&part)) return rc;
+    const int nb = std::max(1, (n + kRedTile - 1) / kRedTile);
+    hipLaunchKernelGGL(k_reduce_partial<0>, dim3(nb), dim3(kRedThreads), 0, stream, n, x,
+                       (const double*)nullptr, part);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, stream, nb, (const double*)part,
+                       out);
+    return launch_check("pt_reduce_sum_f64_noalloc");
 }
 
 extern "C" int pt_kinetic_energy(int n, const double* vel, const double* mass, double* out,

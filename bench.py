@@ -5,19 +5,26 @@ Workload: 32,768 H2O (98,304 atoms) in a 187.7 Bohr periodic box, velocity
 Verlet with a = F/m, F = -grad E, dt = 0.5 fs, Berendsen 300 K, skin
 neighbour list.  Teacher NNP: 52 ACSF per element (tanh cutoff 12 Bohr),
 MLP 52-64-64-1 tanh, weights ~ N(0, 1/fan_in), min-max scaler fitted on the
-structure.  Legs, all on the same structure and protocol:
+structure.  Every step runs without a host read (skin check and rebuild on
+the device) and is replayed from a HIP graph.  Legs, all on the same
+structure and protocol:
 
   surrogate (headline): the trained student (34 ACSF, MLP 34-16-16-1, mixed
-      precision) drives; the FP64 teacher judges every 20 steps (every 5
-      while the student is on probation: until its first pass and after a
-      failure); a force RMSE above
-      5 meV/Bohr rewinds to the last good state and the mixed-precision
-      teacher drives the next 200 steps, judged the same way (a failure there
-      hands the window to the FP64 teacher).
+      precision) drives; the FP64 teacher judges every 20 steps on the
+      neighbourhoods of 128 sampled atoms (every 5 steps while the student is
+      on probation: until its first pass and after a failure); a force RMSE
+      above 5 meV/Bohr rewinds to the last good state and the
+      mixed-precision teacher drives the next 200 steps, judged the same way
+      (a failure there hands the window to the FP64 teacher).
+  surrogate at the student's accuracy: the same with the threshold at twice
+      the student's force RMSE, i.e. what the learned part buys when it drives.
   teacher mixed: descriptors + adjoint FP32, MLP split-BF16 on MFMA, FP64
       energy sum, force gather and integrator.
   teacher FP64: the plain FP64 GPU path.
-The CPU baseline is the FP64 oracle on a bounded sample of centres.
+  student alone: the student driving without a judge.
+CPU baselines: the FP64 oracle with a cell list (headline structure and two
+smaller boxes, sampled centres) and the reference's dense algorithm on an
+N-ladder.
 
 One step = position update, neighbour maintenance, one full energy + force
 evaluation, velocity update and thermostat; inputs are resident in HBM.
@@ -264,7 +271,7 @@ def run_md(pot, s, precision, steps, warmup, args, instrument=False):
     from pantea_amd.units import units
 
     sysm = System.from_structure(s, pot, temperature=300.0, seed=2023, force_mode="full",
-                                 precision=precision, skin=args.skin)
+                                 precision=precision, skin=args.skin, graph=not args.no_graph and not instrument)
     md = MDSimulator(time_step=0.5 * units.FROM_FEMTO_SECOND, thermostat=_thermostat(args), integrator="physical")
     md.run(sysm, warmup)
     _sync()
@@ -278,15 +285,16 @@ def run_md(pot, s, precision, steps, warmup, args, instrument=False):
     return dt, timers, sysm
 
 
-def run_surrogate(teacher, student, s, steps, warmup, args, instrument=False):
+def run_surrogate(teacher, student, s, steps, warmup, args, instrument=False, threshold_mev=None):
     from pantea_amd.surrogate import SurrogateMD
     from pantea_amd.units import units
 
+    thr = args.threshold_mev if threshold_mev is None else threshold_mev
     sur = SurrogateMD(teacher, student, s, seed=2023, judge_every=args.judge_every,
-                      threshold=args.threshold_mev / units.TO_MILLI_ELECTRON_VOLT,
+                      threshold=thr / units.TO_MILLI_ELECTRON_VOLT,
                       fallback_steps=args.fallback_steps, fallback_precision=args.fallback_precision,
                       skin=args.skin, thermostat=_thermostat(args), judge_atoms=args.judge_atoms or None,
-                      probation_every=args.probation_every)
+                      probation_every=args.probation_every, graph=not args.no_graph and not instrument)
     sur.run(warmup)
     _sync()
     before = sur.stats.as_dict()
@@ -311,6 +319,8 @@ def run_surrogate(teacher, student, s, steps, warmup, args, instrument=False):
     delta["student_judge_rmse_mev_per_bohr_min"] = float(np.min(rs)) if rs.size else None
     delta["fallback_judges"] = int(rf.size)
     delta["fallback_judge_rmse_mev_per_bohr_max"] = float(np.max(rf)) if rf.size else None
+    delta["threshold_mev_per_bohr"] = thr
+    delta["judge_centres"] = getattr(sur, "judge_centres", None)
     return dt, timers, delta
 
 
@@ -330,6 +340,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--teacher-steps", type=int, default=100, help="timed steps of the mixed teacher leg")
     ap.add_argument("--fp64-steps", type=int, default=60)
+    ap.add_argument("--sweep-steps", type=int, default=200,
+                    help="timed steps of the surrogate at the student's own accuracy level (0: skip)")
     ap.add_argument("--skin", type=float, default=1.0)
     ap.add_argument("--tau-fs", type=float, default=0.5, help="Berendsen time constant (0: NVE)")
     ap.add_argument("--judge-every", type=int, default=20)
@@ -339,11 +351,12 @@ def main() -> int:
     ap.add_argument("--fallback-precision", default="mixed", choices=("mixed", "float64"),
                     help="precision of the fallback teacher (below the judge's it is judged too)")
     ap.add_argument("--threshold-mev", type=float, default=5.0)
-    ap.add_argument("--judge-atoms", type=int, default=0,
+    ap.add_argument("--judge-atoms", type=int, default=128,
                     help="judge on this many sampled atoms (their neighbourhoods only); 0: every atom")
     ap.add_argument("--molecules", type=int, default=32768)
     ap.add_argument("--box", type=float, default=187.7)
     ap.add_argument("--student", default=None, help="student directory (default: the packaged water student)")
+    ap.add_argument("--no-graph", action="store_true", help="launch every step from the host (no HIP graphs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -386,14 +399,14 @@ def main() -> int:
         es, fs = student.energy_and_forces(s, mode="full", precision="mixed")
         _sync()
         stud_acc = {"energy_error_mev_per_atom": abs(float(es) - float(e64)) / n * mev,
-                    "force_rmse_mev_per_bohr": float(torch.sqrt(torch.mean((fs - f64) ** 2))) * mev}
+                    "force_rmse_mev_per_bohr": float(torch.sqrt(torch.mean((fs - f64) ** 2))) * mev,
+                    "r_cutoff_bohr": student.r_cutoff}
         _log(f"[bench] student vs FP64 teacher on the start structure: {stud_acc}")
 
     # teacher legs
     dt_m, _, _ = run_md(teacher, s, "mixed", args.teacher_steps, args.warmup, args)
     mixed_value = n * args.teacher_steps / dt_m
     _log(f"[bench] teacher mixed MD: {1e3 * dt_m / args.teacher_steps:.3f} ms/step, {mixed_value:.4g} atom-steps/s")
-    # per-phase breakdown from an instrumented repeat
     phase_steps_m = min(args.teacher_steps, 50)
     _, timers_m, _ = run_md(teacher, s, "mixed", phase_steps_m, 2, args, instrument=True)
     phases_m = phase_table(timers_m)
@@ -403,7 +416,7 @@ def main() -> int:
         fp64_value = n * args.fp64_steps / dt64
         _log(f"[bench] teacher FP64 MD: {1e3 * dt64 / args.fp64_steps:.3f} ms/step, {fp64_value:.4g} atom-steps/s")
 
-    # the student alone (no judge): the rate the surrogate would reach if every judge passed
+    # the student alone (no judge): the rate the surrogate reaches when every judge passes
     student_value = None
     if student is not None:
         dt_st, _, _ = run_md(student, s, "mixed", args.teacher_steps, args.warmup, args)
@@ -411,33 +424,47 @@ def main() -> int:
         _log(f"[bench] student mixed MD (no judge): {1e3 * dt_st / args.teacher_steps:.3f} ms/step, "
              f"{student_value:.4g} atom-steps/s")
 
-    # headline leg
+    # headline leg: the configured surrogate (threshold as given)
     sur = None
+    sweep = None
     if student is not None:
         dt_s, _, sur = run_surrogate(teacher, student, s, args.steps, args.warmup, args)
         value = n * args.steps / dt_s
         ms_per_step = 1e3 * dt_s / args.steps
         steps = args.steps
         _log(f"[bench] surrogate MD: {ms_per_step:.3f} ms/step, {value:.4g} atom-steps/s, {sur}")
-        # phases and the roofline kernel's launch time from an instrumented repeat
         phase_steps = min(args.steps, 200)
         _, timers_s, _ = run_surrogate(teacher, student, s, phase_steps, args.warmup, args, instrument=True)
         phases = phase_table(timers_s)
+        # the same surrogate with the judge's threshold at the student's own accuracy level
+        # (twice its force RMSE on the start structure): what the learned part buys when it drives
+        if args.sweep_steps > 0 and stud_acc is not None:
+            thr = 2.0 * stud_acc["force_rmse_mev_per_bohr"]
+            dt_w, _, sw = run_surrogate(teacher, student, s, args.sweep_steps, args.warmup, args, threshold_mev=thr)
+            sweep = {"threshold_mev_per_bohr": thr, "steps": args.sweep_steps,
+                     "atom_steps_per_s": n * args.sweep_steps / dt_w, "ms_per_step": 1e3 * dt_w / args.sweep_steps,
+                     "over_fp64": (n * args.sweep_steps / dt_w / fp64_value) if fp64_value else None,
+                     "surrogate": sw}
+            _log(f"[bench] surrogate at {thr:.4g} meV/Bohr: {sweep['ms_per_step']:.3f} ms/step, {sw}")
     else:
         value, ms_per_step, phases, steps = mixed_value, 1e3 * dt_m / args.teacher_steps, phases_m, args.teacher_steps
         phase_steps = phase_steps_m
 
-    # roofline of the kernel with the most time in the headline run
+    # roofline of the descriptor kernel with the most time in the headline run
     n_nb, n_trip = neighbour_stats(s, teacher.r_cutoff)
     cands = {k: v for k, v in phases.items() if k.rsplit("/", 1)[-1] in ("acsf_forward", "acsf_adjoint")}
     dom = max(cands, key=lambda k: cands[k][0])
     label, ph = (dom.rsplit("/", 1) if "/" in dom else ("teacher-mixed", dom))
     is_student = label.startswith("student")
     fp64 = label.endswith("float64")
-    fwd_ops, adj_ops = algorithmic_ops(n_nb, n_trip, 8, 6 if is_student else 12)
+    pot_of = student if is_student else teacher
+    if is_student:
+        nb_s, tr_s = neighbour_stats(s, student.r_cutoff)
+        fwd_ops, adj_ops = algorithmic_ops(nb_s, tr_s, 8, 6, quartic=False)
+    else:
+        fwd_ops, adj_ops = algorithmic_ops(n_nb, n_trip, 8, 12, quartic=True)
     mode = 1 if ph == "acsf_adjoint" else 0
     total_ms, groups = phases[dom]
-    pot_of = student if is_student else teacher
     # elements with byte-identical descriptor tables share one launch per evaluation
     n_launch = len({bytes(pot_of.atomic_potentials[e].descriptor.device_table(s.element_map, s.device).host)
                     for e in pot_of.elements})
@@ -445,7 +472,6 @@ def main() -> int:
     launch_ops = (adj_ops if mode else fwd_ops) * n / n_launch
     achieved = launch_ops / (launch_ms * 1e-3) / 1e12 if launch_ms > 0 else 0.0
     peak = PEAK_FP64_TFLOPS if fp64 else PEAK_FP32_TFLOPS
-    # the launch's kernel: table class (pt_sf_table_kind) and the four-/eight-wave forward variants
     kind = pot_of.atomic_potentials["O"].descriptor.device_table(s.element_map, s.device).kind
     if mode == 0 and fp64 and kind != 0:
         variant = "_w4"
@@ -455,7 +481,7 @@ def main() -> int:
         variant = ""
     kname = f"k_acsf{variant}<{'double' if fp64 else 'float'}, {mode}, {kind}>"
     roofline = {
-        "bound": "mfma",
+        "bound": "valu",
         "achieved": achieved,
         "peak": peak,
         "unit": "TFLOP/s",
@@ -464,22 +490,21 @@ def main() -> int:
         "kernel": f"{kname} ({'descriptor adjoint' if mode else 'descriptor forward'}, {label})",
         "avg_launch_ms": launch_ms,
         "algorithmic_ops_per_launch": launch_ops,
-        "note": "vector-ALU bound work (no MFMA in this kernel; 'mfma' is the compute roofline): "
-                f"algorithmic ops per atom {('2.5 x ' if mode else '')}((15 + 6 g2) n_nb + (26 + 5 g3) n_trip), "
-                f"g2 = 8, g3 = {6 if is_student else 12} per neighbour pair type, transcendental = 1, FMA = 2; "
-                f"measured n_nb={n_nb:.1f}, n_trip={n_trip:.0f} per atom; a launch covers "
-                f"{'all' if n_launch == 1 else f'1/{n_launch} of the'} centres (the O and H tables are "
-                f"{'identical' if n_launch == 1 else 'different'}); average launch time from HIP events over "
-                f"{groups * n_launch} launches of an "
-                f"instrumented {phase_steps}-step repeat of the headline run; peak = published MI355X {'FP64' if fp64 else 'FP32'} vector; traffic = HBM bytes "
-                "per launch from the FETCH_SIZE/WRITE_SIZE profiles, copy-kernel calibrated",
+        "note": "vector-ALU bound (no matrix cores in this kernel): algorithmic ops per centre, forward "
+                "(15 + 6 g2) n_nb + (26 + 5 g3) n_trip, adjoint counted from its neighbour and pair bodies "
+                "(algorithmic_ops in bench.py); transcendental = 1, FMA = 2; "
+                f"measured n_nb={n_nb:.1f}, n_trip={n_trip:.0f} per atom at the teacher's cutoff; a launch covers "
+                f"{'all' if n_launch == 1 else f'1/{n_launch} of the'} centres; average launch time from HIP events "
+                f"over {groups * n_launch} launches of an instrumented {phase_steps}-step repeat of the headline run; "
+                f"peak = published MI355X {'FP64' if fp64 else 'FP32'} vector; traffic = HBM bytes per launch from "
+                "the FETCH_SIZE/WRITE_SIZE profile, copy-kernel calibrated",
     }
     _log(f"[bench] roofline: {roofline['kernel']} {achieved:.3g} TFLOP/s = {roofline['frac']:.3f} of peak")
 
-    cpu = None
+    cpu, cpu_legs = None, None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(teacher, s)
+            cpu, cpu_legs = cpu_baselines(teacher, args.molecules, args.box, _oracle_threads())
             _log(f"[bench] cpu baseline: {cpu['value']:.4g} atom-steps/s on {cpu['cores']} threads")
         except Exception as exc:  # the baseline is a report, not the measurement
             _log(f"[bench] cpu baseline failed: {exc!r}")
@@ -498,9 +523,18 @@ def main() -> int:
     judged = f"{args.judge_atoms} sampled atoms" if args.judge_atoms else "all atoms"
     fb = ("mixed-precision teacher steps (judged the same way; failing -> FP64 teacher)"
           if args.fallback_precision == "mixed" else "FP64 teacher steps")
-    head = ("surrogate: student 34 ACSF 34-16-16-1 mixed drives, FP64 teacher judges every "
-            f"{args.judge_every} steps (a student on probation, until it passes: every {args.probation_every}) on {judged}, "
-            f"> {args.threshold_mev} meV/Bohr -> rewind + {args.fallback_steps} {fb}") if sur is not None else "mixed-precision teacher (no trained student found)"
+    drove = bool(sur and sur["student_steps"] > 0)
+    if sur is None:
+        head = "mixed-precision teacher (no trained student found)"
+    else:
+        head = (f"surrogate: student 34 ACSF (rc {student.r_cutoff} Bohr) 34-16-16-1 mixed drives, FP64 teacher "
+                f"judges every {args.judge_every} steps (a student on probation, until it passes: every "
+                f"{args.probation_every}) on {judged}, > {args.threshold_mev} meV/Bohr -> rewind + "
+                f"{args.fallback_steps} {fb}")
+        if not drove:
+            head += ("; in the timed window the student drove no step (it fails the judge), so the value is the "
+                     "judged mixed-precision teacher's rate: the surrogate's own rate at this threshold is "
+                     "UNMEASURED (see surrogate_at_student_accuracy)")
     out = {
         "metric": "atom-steps/s, 98k-atom water NNP-MD: mixed+surrogate vs FP64; force RMSE",
         "value": value,
@@ -517,7 +551,10 @@ def main() -> int:
                 "student trained on the GPU on teacher MD data",
         "config": {"workload": f"water NNP-MD, {n} atoms ({n // 3} H2O), box {args.box} Bohr, rc 12 Bohr, "
                                f"teacher 52 ACSF/element + MLP 52-64-64-1, dt 0.5 fs, {thermo}; headline: {head}",
-                   "atoms": n, "skin_bohr": args.skin, "parallelism": "single GPU"},
+                   "atoms": n, "skin_bohr": args.skin, "parallelism": "single GPU",
+                   "step_launch": "host-free step (device skin check + gated rebuild)"
+                                  + ("" if args.no_graph else ", replayed from a HIP graph")},
+        "headline_driver": ("student" if drove else "teacher (mixed precision, judged)") if sur else "teacher",
         "fp64_atom_steps_per_s": fp64_value,
         "mixed_teacher_atom_steps_per_s": mixed_value,
         "student_mixed_atom_steps_per_s": student_value,
@@ -525,6 +562,7 @@ def main() -> int:
         "value_over_fp64": (value / fp64_value) if fp64_value else None,
         "mixed_teacher_over_fp64": (mixed_value / fp64_value) if fp64_value else None,
         "surrogate": sur,
+        "surrogate_at_student_accuracy": sweep,
         "student_accuracy_vs_fp64": stud_acc,
         "energy_error_mev_per_atom": e_err,
         "force_rmse_mev_per_bohr": f_rmse,
@@ -534,10 +572,11 @@ def main() -> int:
         "force_rms_mev_per_bohr": f_rms,
         "phase_ms_per_step": {k: v[0] / phase_steps for k, v in phases.items()},
         "phase_note": f"HIP events of instrumented repeats ({phase_steps} headline steps, {phase_steps_m} mixed "
-                      "teacher steps); the timed runs carry no events",
+                      "teacher steps, launched from the host); the timed runs carry no events",
         "teacher_mixed_phase_ms_per_step": {k: v[0] / phase_steps_m for k, v in phases_m.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_legs": cpu_legs,
     }
     print(json.dumps(out), flush=True)
     return 0

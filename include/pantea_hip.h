@@ -218,6 +218,20 @@ int pt_acsf_adjoint(int precision, const double *pos4, const double *box,
                     const pt_sf_table *table, const void *w, int ldw,
                     double *f_self, void *edge_grad, pt_stream_t stream);
 
+/* The adjoint with a transposed edge buffer: edge e(i,j)'s gradient
+ * sum_s w_is dG_is/dr_j goes to the slot of the reverse edge, rev[e]
+ * (device, from pt_nbr_build), so that edge_grad_t[e(j,i)] = dE_i/dr_j and
+ * row j of the buffer holds everything atom j receives.  Every slot whose
+ * reverse edge belongs to a launched centre is written (zero beyond the
+ * cutoff); f_self as in pt_acsf_adjoint.  Same forces as pt_acsf_adjoint +
+ * pt_force_gather, read contiguously by pt_force_gather_transposed; no
+ * counterpart in the reference (force.py:16-43 is autodiff). */
+int pt_acsf_adjoint_transposed(int precision, const double *pos4, const double *box,
+                               const long long *row_off, const int *nbr, const int *rev, int max_row,
+                               int n_centers, int center_begin, const int *centers,
+                               const pt_sf_table *table, const void *w, int ldw,
+                               double *f_self, void *edge_grad_t, pt_stream_t stream);
+
 /* Full forces F = -grad E by a scatter-free gather over the edge buffer:
  *   F[i] = sum_{e in row i} (edge_grad[e] - edge_grad[rev[e]]).
  * (FP64 accumulation; edge_precision PT_F64 or PT_F32.)  Replaces the
@@ -225,6 +239,12 @@ int pt_acsf_adjoint(int precision, const double *pos4, const double *box,
 int pt_force_gather(int n, const long long *row_off, const int *rev,
                     const void *edge_grad, int edge_precision, double *f_full,
                     pt_stream_t stream);
+/* F[i] = f_self[i] - sum_{e in row i} edge_grad_t[e] over the transposed
+ * buffer of pt_acsf_adjoint_transposed (f_self: device, n x 3 FP64, the
+ * adjoint's own output): one contiguous read per row. */
+int pt_force_gather_transposed(int n, const long long *row_off, const double *f_self,
+                               const void *edge_grad_t, int edge_precision, double *f_full,
+                               pt_stream_t stream);
 
 /* ------------------------------------------------------------------------
  * Descriptor tangent: T[c, s] = sum_m dG_{i,s}/dr_m . delta[m] for centre

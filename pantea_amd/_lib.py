@@ -110,7 +110,10 @@ _SIGS = {
                                  _vp, _c_int, _vp, _vp, _vp]),
     "pt_acsf_tangent": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp,
                                  _vp, _c_int, _vp, _c_int, _vp]),
+    "pt_acsf_adjoint_transposed": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp,
+                                            _vp, _c_int, _vp, _vp, _vp]),
     "pt_force_gather": (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _vp, _vp]),
+    "pt_force_gather_transposed": (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _vp, _vp]),
     "pt_mlp_packed_bytes": (_c_size, [_vp]),
     "pt_mlp_pack": (_c_int, [_vp, _vp, _vp, _vp]),
     "pt_mlp_energy_grad": (_c_int, [_c_int, _c_int, _vp, _c_int, _c_int, _vp, _vp, _vp, _vp,

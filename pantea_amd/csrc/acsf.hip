@@ -354,6 +354,7 @@ struct KArgs {
     int ldw;
     double* f_self;       // ADJ (n_centers x 3)
     void* edge_grad;      // ADJ (n_pairs x 3)
+    const int* rev;       // ADJ: when set, edge e's gradient goes to slot rev[e] (the reverse edge)
     const double* delta;  // TAN (n x 3, internal order)
     int center_only;
     int cap;
@@ -748,8 +749,11 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
             for (int q = lane; q < a.ld; q += kWave) out[(size_t)c * a.ld + q] = nan;
         } else {
             if (a.edge_grad) {
-                R* eg = (R*)a.edge_grad + 3 * off;
-                for (int t = lane; t < 3 * len; t += kWave) eg[t] = nan;
+                R* eg = (R*)a.edge_grad;
+                for (int t = lane; t < len; t += kWave) {
+                    const long long e = a.rev ? (long long)a.rev[off + t] : off + t;
+                    if (e >= 0) { eg[3 * e] = nan; eg[3 * e + 1] = nan; eg[3 * e + 2] = nan; }
+                }
             }
             if (a.f_self && lane < 3) a.f_self[3 * c + lane] = (double)nan;
         }
@@ -816,8 +820,12 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
                 S.D[s] = (R)ex; S.D[cap + s] = (R)ey; S.D[2 * cap + s] = (R)ez;
             }
         } else if (MODE == MODE_ADJ && t < len && a.edge_grad) {
-            R* eg = (R*)a.edge_grad + 3 * (off + t);
-            eg[0] = R(0); eg[1] = R(0); eg[2] = R(0);
+            // (a reverse index of -1 would mean an asymmetric list: no slot to write)
+            const long long e = a.rev ? (long long)a.rev[off + t] : off + t;
+            if (e >= 0) {
+                R* eg = (R*)a.edge_grad + 3 * e;
+                eg[0] = R(0); eg[1] = R(0); eg[2] = R(0);
+            }
         }
         m += __popcll(bal);
         const int tk = (int)tj;
@@ -1013,10 +1021,12 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
                 const unsigned long long bal = S.keep[t0 / kWave];
                 if ((bal >> lane) & 1ull) {
                     const int s = ms + lanes_below(bal);
-                    const long long e = off + t0 + lane;
-                    eg[3 * e + 0] = bad ? nan : (R)S.DA[3 * s];
-                    eg[3 * e + 1] = bad ? nan : (R)S.DA[3 * s + 1];
-                    eg[3 * e + 2] = bad ? nan : (R)S.DA[3 * s + 2];
+                    const long long e = a.rev ? (long long)a.rev[off + t0 + lane] : off + t0 + lane;
+                    if (e >= 0) {
+                        eg[3 * e + 0] = bad ? nan : (R)S.DA[3 * s];
+                        eg[3 * e + 1] = bad ? nan : (R)S.DA[3 * s + 1];
+                        eg[3 * e + 2] = bad ? nan : (R)S.DA[3 * s + 2];
+                    }
                 }
                 ms += __popcll(bal);
             }
@@ -1176,6 +1186,23 @@ extern "C" int pt_acsf_adjoint(int precision, const double* pos4, const double* 
     a.ldw = ldw;
     a.f_self = f_self;
     a.edge_grad = edge_grad;
+    return run(precision, MODE_ADJ, a, st);
+}
+
+extern "C" int pt_acsf_adjoint_transposed(int precision, const double* pos4, const double* box,
+                                          const long long* row_off, const int* nbr, const int* rev, int max_row,
+                                          int n_centers, int center_begin, const int* centers,
+                                          const pt_sf_table* table, const void* w, int ldw, double* f_self,
+                                          void* edge_grad_t, pt_stream_t stream) {
+    PT_REQUIRE(pos4 && row_off && rev && table && (w || n_centers == 0),
+               "pt_acsf_adjoint_transposed: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    KArgs a = common_args(pos4, box, row_off, nbr, max_row, n_centers, center_begin, centers, table);
+    a.w = w;
+    a.ldw = ldw;
+    a.f_self = f_self;
+    a.edge_grad = edge_grad_t;
+    a.rev = rev;
     return run(precision, MODE_ADJ, a, st);
 }
 

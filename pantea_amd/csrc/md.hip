@@ -241,6 +241,31 @@ __global__ __launch_bounds__(256) void k_force_gather(int n, const long long* __
     if (lane == 0) { f[3 * i] = fx; f[3 * i + 1] = fy; f[3 * i + 2] = fz; }
 }
 
+// Transposed edge buffer: gT[e(i->j)] = dE_j/dr_i (written by centre j at the
+// reverse slot), so F_i = f_self_i - sum_{e in row i} gT[e] reads row i
+// contiguously, without the reverse index.
+template <typename E>
+__global__ __launch_bounds__(256) void k_force_gather_t(int n, const long long* __restrict__ row_off,
+                                                        const double* __restrict__ f_self,
+                                                        const E* __restrict__ gT, double* __restrict__ f) {
+    const int lane = wave_lane();
+    const int i = uniform(xcd_block() * 4 + (threadIdx.x >> 6));
+    if (i >= n) return;
+    const long long b = row_off[i], e = row_off[i + 1];
+    double fx = 0, fy = 0, fz = 0;
+    for (long long k = b + lane; k < e; k += kWave) {
+        fx += (double)gT[3 * k];
+        fy += (double)gT[3 * k + 1];
+        fz += (double)gT[3 * k + 2];
+    }
+    fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
+    if (lane == 0) {
+        f[3 * i] = f_self[3 * i] - fx;
+        f[3 * i + 1] = f_self[3 * i + 1] - fy;
+        f[3 * i + 2] = f_self[3 * i + 2] - fz;
+    }
+}
+
 inline int nblocks(long long work, int threads) { return (int)((work + threads - 1) / threads); }
 
 // ---- gated state updates of an MD step that never waits on the host ----------
@@ -488,6 +513,25 @@ extern "C" int pt_gather_rows_i32(int n, int width, const int* idx, const int* s
     hipLaunchKernelGGL(k_gather<int>, dim3(nblocks((long long)n * width, 256)), dim3(256), 0,
                        (hipStream_t)stream, n, width, idx, src, dst);
     return launch_check("pt_gather_rows_i32");
+}
+
+extern "C" int pt_force_gather_transposed(int n, const long long* row_off, const double* f_self,
+                                          const void* edge_grad_t, int edge_precision, double* f_full,
+                                          pt_stream_t stream) {
+    PT_REQUIRE(n >= 0 && row_off && f_self && edge_grad_t && f_full, "pt_force_gather_transposed: bad arguments");
+    if (n == 0) return PT_OK;
+    const int blocks = (n + 3) / 4;
+    if (edge_precision == PT_F64)
+        hipLaunchKernelGGL(k_force_gather_t<double>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, n,
+                           row_off, f_self, (const double*)edge_grad_t, f_full);
+    else if (edge_precision == PT_F32)
+        hipLaunchKernelGGL(k_force_gather_t<float>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, n,
+                           row_off, f_self, (const float*)edge_grad_t, f_full);
+    else {
+        set_error("pt_force_gather_transposed: unknown precision %d", edge_precision);
+        return PT_ERR_ARG;
+    }
+    return launch_check("pt_force_gather_transposed");
 }
 
 extern "C" int pt_force_gather(int n, const long long* row_off, const int* rev, const void* edge_grad,

@@ -299,8 +299,16 @@ def acsf_forward(nl: NeighborList, table: DeviceTable, precision: int, n_centers
 
 
 def acsf_adjoint(nl: NeighborList, table: DeviceTable, precision: int, n_centers: int, w, ldw: int,
-                 center_begin: int = 0, centers=None, f_self=None, edge_grad=None):
+                 center_begin: int = 0, centers=None, f_self=None, edge_grad=None, transposed: bool = False):
+    """transposed: edge gradients go to the reverse edge's slot (pt_acsf_adjoint_transposed),
+    for force_gather_transposed."""
     box_p, _keep = _box_arg(nl.box_lengths)
+    if transposed:
+        _lib.check(_lib.load().pt_acsf_adjoint_transposed(
+            precision | (table.kind << _lib.PT_ACSF_KIND_SHIFT), nl.pos4.data_ptr(), box_p, nl.row_off.data_ptr(),
+            nl.nbr.data_ptr(), nl.rev.data_ptr(), nl.max_row, n_centers, center_begin, _vp(centers), table.ptr,
+            w.data_ptr(), ldw, _vp(f_self), _vp(edge_grad), _lib.stream_ptr()), "pt_acsf_adjoint_transposed")
+        return
     _lib.check(_lib.load().pt_acsf_adjoint(
         precision | (table.kind << _lib.PT_ACSF_KIND_SHIFT), nl.pos4.data_ptr(), box_p, nl.row_off.data_ptr(), nl.nbr.data_ptr(), nl.max_row,
         n_centers, center_begin, _vp(centers), table.ptr, w.data_ptr(), ldw, _vp(f_self),
@@ -329,6 +337,17 @@ def force_gather(nl: NeighborList, edge_grad, precision: int, out=None):
     _lib.check(_lib.load().pt_force_gather(nl.n, nl.row_off.data_ptr(), nl.rev.data_ptr(),
                                            edge_grad.data_ptr(), precision, out.data_ptr(),
                                            _lib.stream_ptr()), "pt_force_gather")
+    return out
+
+
+def force_gather_transposed(nl: NeighborList, edge_grad_t, precision: int, f_self, out=None):
+    """F = f_self - row sums of the transposed edge buffer (pt_force_gather_transposed)."""
+    torch = _torch()
+    if out is None:
+        out = torch.empty((nl.n, 3), dtype=torch.float64, device=nl.pos4.device)
+    _lib.check(_lib.load().pt_force_gather_transposed(nl.n, nl.row_off.data_ptr(), f_self.data_ptr(),
+                                                      edge_grad_t.data_ptr(), precision, out.data_ptr(),
+                                                      _lib.stream_ptr()), "pt_force_gather_transposed")
     return out
 
 

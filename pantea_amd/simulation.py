@@ -36,7 +36,8 @@ import numpy as np
 
 from . import _lib
 from .atoms import ElementMap, host_box
-from .engine import NeighborList, Reducer, acsf_adjoint, acsf_forward, force_gather, gather_rows, scatter_rows
+from .engine import (NeighborList, Reducer, acsf_adjoint, acsf_forward, force_gather, force_gather_transposed,
+                     gather_rows, scatter_rows)
 from .potential import PRECISION_MODES, NeuralNetworkPotential
 from .units import units
 
@@ -136,12 +137,13 @@ class _ForceField:
             target.zero_()
         mark("acsf_adjoint", True)
         for tab, b, cnt in runs:
+            # -grad E: edge gradients in the reverse edges' slots, so the gather reads rows contiguously
             acsf_adjoint(nl, tab, self.g_prec, cnt, self.dEdG[b:], self.ld, center_begin=b,
-                         f_self=target[b:], edge_grad=self.edge if full else None)
+                         f_self=target[b:], edge_grad=self.edge if full else None, transposed=full)
         mark("acsf_adjoint", False)
         if full:
             mark("force_gather", True)
-            force_gather(nl, self.edge, self.g_prec, out=out)
+            force_gather_transposed(nl, self.edge, self.g_prec, self.f_self, out=out)
             mark("force_gather", False)
         return out
 
@@ -190,6 +192,9 @@ class _DeviceMD:
         self.timers = None  # {phase: [(start_event, end_event), ...]} when profiling
         self._builds = 0
         self.redos = 0        # stretches redone after a list overflow
+        self.use_graph = False
+        self._graph = None
+        self._graph_key = None
         self._since = 0       # gated steps since the last verified checkpoint
         self._ck = None
         self._args = None
@@ -388,6 +393,39 @@ class _DeviceMD:
 
     # ---- one step ------------------------------------------------------------
     def step(self, dt: float, integrator: str, force_mode: str, thermostat: Optional[BrendsenThermostat]):
+        """One MD step.  With ``use_graph`` (gated steps only), the step's launch sequence is
+        captured once into a HIP graph and replayed; it is captured again whenever a buffer
+        or a parameter of the step changes."""
+        args = (dt, integrator, force_mode, thermostat)
+        if not (self.gated and self.use_graph) or self.timers is not None:
+            self._graph = None
+            self._step_body(*args, swap=True)
+            return
+        import torch
+
+        if self._graph is not None and self._graph_key == self._key(args):
+            self._graph.replay()
+            return
+        self._graph = None
+        self._step_body(*args, swap=False)      # this step, run eagerly (sizes every buffer)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._step_body(*args, swap=False)  # recorded, not run
+        self._graph, self._graph_key = g, self._key(args)
+
+    def _key(self, args) -> tuple:
+        """Everything a captured step depends on: its parameters and every buffer address."""
+        ff = self.ff
+        nl = self._nl
+        bufs = (nl.pos4, nl.perm, nl.row_off, nl.nbr, nl.rev, nl.ws, self.vel, self.force, self.force_new,
+                self.mass, self.inv_mass, self.orig, self.types, self.ref_pos, self.pos_in, self.tmp3, self.tmp1,
+                self.tmpi, self.flag, self.totals, self.disp, self.reducer.ws, self.reducer.out, ff.G, ff.dEdG,
+                ff.e_atom, ff.f_self, ff.edge)
+        return (args[:3], id(args[3]), id(ff), id(ff.mlps), nl.cap, self.row_bound,
+                tuple(0 if b is None else b.data_ptr() for b in bufs))
+
+    def _step_body(self, dt: float, integrator: str, force_mode: str, thermostat: Optional[BrendsenThermostat],
+                   swap: bool):
         lib = _lib.load()
         inv_m = self.inv_mass.data_ptr() if integrator == "physical" else None
         box = host_box(self.box)
@@ -407,7 +445,10 @@ class _DeviceMD:
         _lib.check(lib.pt_verlet_velocities(self.n, self.vel.data_ptr(), self.force.data_ptr(),
                                             self.force_new.data_ptr(), inv_m, dt, _lib.stream_ptr()),
                    "pt_verlet_velocities")
-        self.force, self.force_new = self.force_new, self.force
+        if swap:
+            self.force, self.force_new = self.force_new, self.force
+        else:   # a replayed step keeps its buffers
+            self.force.copy_(self.force_new)
         if thermostat is not None:
             ke = self.reducer.kinetic(self.vel, self.mass)
             _lib.check(lib.pt_berendsen_rescale(self.n, self.vel.data_ptr(), ke.data_ptr(), dt,
@@ -466,10 +507,11 @@ class System:
 
     def __init__(self, potential: NeuralNetworkPotential, structure, velocities, masses,
                  force_mode: str = "reference", precision: str = "float64", skin: float = 0.0,
-                 sync_free: bool = True):
+                 sync_free: bool = True, graph: bool = False):
         if force_mode not in ("reference", "full"):
             raise ValueError(f"unknown force mode '{force_mode}'")
         self.sync_free = bool(sync_free)
+        self.graph = bool(graph)
         self.potential = potential
         self.structure = structure
         self._velocities = velocities
@@ -500,11 +542,13 @@ class System:
     @classmethod
     def from_structure(cls, structure, potential, temperature: float = 300.0, seed: int = 2024,
                        force_mode: str = "reference", precision: Optional[str] = None,
-                       skin: float = 0.0, sync_free: bool = True) -> "System":
+                       skin: float = 0.0, sync_free: bool = True, graph: bool = False) -> "System":
+        """(system.py:75-96).  sync_free: periodic steps never wait for the host;
+        graph: such steps are replayed from a captured HIP graph."""
         masses = structure.element_map.get_masses_from_structure(structure)
         vel = cls.generate_random_velocities(temperature, masses, seed)
         prec = precision or getattr(potential, "precision", "float64")
-        return cls(potential, structure.copy(), vel, masses, force_mode, prec, skin, sync_free)
+        return cls(potential, structure.copy(), vel, masses, force_mode, prec, skin, sync_free, graph)
 
     @staticmethod
     def generate_random_velocities(temperature: float, masses, seed: int):
@@ -523,6 +567,7 @@ class System:
     def _engine(self, extra=()) -> _DeviceMD:
         if self._md is None:
             self._md = _DeviceMD(self, self.skin, extra, sync_free=self.sync_free)
+            self._md.use_graph = self.graph
         return self._md
 
     def _sync_from_engine(self) -> None:

@@ -23,6 +23,7 @@ defaults (settings.py:116-121) are followed where they apply.
 
 from __future__ import annotations
 
+import json
 import math
 import time
 from dataclasses import dataclass, field
@@ -230,13 +231,18 @@ class StudentTrainer:
             mlp.repack()
         return loss
 
-    def train(self, epochs: int, batch: int = 32, train_idx: Optional[Sequence[int]] = None, log=None):
+    def train(self, epochs: int, batch: int = 32, train_idx: Optional[Sequence[int]] = None, log=None,
+              lr_schedule=None):
+        """``epochs`` passes over ``train_idx`` in shuffled batches; ``lr_schedule(epoch)``
+        (optional) sets the learning rate of each epoch."""
         import torch
 
         idx = np.arange(len(self.snapshots)) if train_idx is None else np.asarray(train_idx)
         history = []
         t0 = time.perf_counter()
         for ep in range(epochs):
+            if lr_schedule is not None:
+                self.lr = float(lr_schedule(ep))
             perm = self.rng.permutation(idx)
             tot = torch.zeros(2, dtype=torch.float64, device=self.dev)
             nb = 0
@@ -279,17 +285,88 @@ class StudentTrainer:
 ASSET_DIR = __import__("pathlib").Path(__file__).resolve().parent / "assets" / "student_water"
 
 
+def rank_teacher_g3(teacher: NeuralNetworkPotential, structures: Sequence[Structure], sample: int = 256,
+                    seed: int = 0) -> Dict[str, Dict[str, List[Tuple[float, Tuple[float, float, float]]]]]:
+    """The teacher's angular functions ranked by how strongly its forces depend on them.
+
+    For central element e and function s: rms over atoms of dE_i/dG_is (the
+    teacher's own gradient w.r.t. its raw descriptors) times rms over a sample of
+    atoms of |dG_is/dr_i| (its self-Jacobian, ACSF.grad): the size of the force
+    component carried by that function.  Returns, per element and neighbour pair
+    ("H-O" ...), [(score, (eta, zeta, lambda)), ...] in decreasing score.
+    """
+    import torch
+
+    from .bench_models import pair_key
+
+    rng = np.random.default_rng(seed)
+    acc: Dict[str, Dict[int, List[float]]] = {}
+    for s in structures:
+        ev = teacher._evaluation(s, "float64")
+        ev.energy(teacher)
+        for e, b, cnt in ev._blocks(teacher):
+            acsf = teacher.atomic_potentials[e].descriptor
+            n_sf = acsf.num_symmetry_functions
+            w2 = (ev.dEdG[b:b + cnt, :n_sf].double() ** 2).sum(0).cpu().numpy()
+            idx = s.select_host(e)
+            pick = np.sort(rng.choice(idx, min(sample, idx.size), replace=False))
+            J = acsf.grad(s, atom_index=pick, precision="float64")
+            j2 = (J.double() ** 2).sum((0, 2)).cpu().numpy()
+            d = acc.setdefault(e, {"w2": np.zeros(n_sf), "nw": 0, "j2": np.zeros(n_sf), "nj": 0})
+            d["w2"] += w2
+            d["nw"] += cnt
+            d["j2"] += j2
+            d["nj"] += pick.size
+    out: Dict[str, Dict[str, List]] = {}
+    for e, d in acc.items():
+        score = np.sqrt(d["w2"] / max(d["nw"], 1)) * np.sqrt(d["j2"] / max(d["nj"], 1))
+        acsf = teacher.atomic_potentials[e].descriptor
+        nr = acsf.num_radial_symmetry_functions
+        per: Dict[str, List] = {}
+        for q, (sf, ne) in enumerate(acsf.angular_symmetry_functions):
+            per.setdefault(pair_key(ne.neighbor_j, ne.neighbor_k), []).append(
+                (float(score[nr + q]), (float(sf.eta), float(sf.zeta), float(sf.lambda0))))
+        out[e] = {p: sorted(v, key=lambda t: -t[0]) for p, v in per.items()}
+    return out
+
+
+def select_g3(ranking, per_pair: int = 6) -> Dict[str, Dict[str, List[Tuple[float, float, float]]]]:
+    """The ``per_pair`` best-ranked cells of every pair, in the teacher's column order."""
+    from .bench_models import G3_GRID
+
+    order = {c: q for q, c in enumerate(G3_GRID)}
+    return {e: {p: sorted((c for _, c in v[:per_pair]), key=lambda c: order.get(tuple(c), 0)) for p, v in d.items()}
+            for e, d in ranking.items()}
+
+
+def cosine_schedule(lr: float, epochs: int, hold: float = 0.5, lr_min: float = 1e-5):
+    """lr for the first ``hold`` of the epochs, then a cosine decay to lr_min."""
+    h = int(hold * epochs)
+
+    def f(ep: int) -> float:
+        if ep < h:
+            return lr
+        t = (ep - h) / max(1, epochs - h - 1)
+        return lr_min + 0.5 * (lr - lr_min) * (1.0 + math.cos(math.pi * min(1.0, t)))
+    return f
+
+
 def train_water_student(out_dir=ASSET_DIR, n_snapshots: int = 2000, n_structures: int = 64, every: int = 10,
-                        epochs: int = 200, batch: int = 32, lr: float = 1e-3, w_force: float = 10.0,
-                        seed: int = 2023, log=print) -> dict:
+                        epochs: int = 600, batch: int = 32, lr: float = 1e-3, w_force: float = 10.0,
+                        seed: int = 2023, r_cutoff: float = 12.0, select: str = "sensitivity",
+                        data=None, log=print) -> dict:
     """Generate teacher data, train the water student on the GPU and save it to ``out_dir``.
 
     Teacher: the 98,304-atom benchmark NNP (52 ACSF, 52-64-64-1); data: its own
     300 K MD on ``n_structures`` water boxes of 32..512 molecules at 1 g/cm^3,
     ``n_snapshots`` snapshots every ``every`` steps with FP64 labels; student:
-    34 ACSF, 34-16-16-1, mixed precision, Adam, loss E-MSE/atom + w_force F-MSE.
+    34 ACSF at cutoff ``r_cutoff`` whose 6 G3 per pair are the teacher's most
+    force-relevant ones (``select="sensitivity"``, rank_teacher_g3 on the first
+    snapshots) or its zeta = 1 ones (``"zeta1"``), MLP 34-16-16-1, mixed
+    precision, Adam (lr for half the epochs, then a cosine decay), loss
+    E-MSE/atom + w_force F-MSE.  ``data``: labelled (structure, E, F) triples
+    to reuse instead of generating them.
     """
-    import json
     from pathlib import Path
 
     from . import bench_models
@@ -297,12 +374,20 @@ def train_water_student(out_dir=ASSET_DIR, n_snapshots: int = 2000, n_structures
     t0 = time.perf_counter()
     _, teacher = bench_models.water_teacher(precision="mixed", seed=seed)
     log(f"[student] teacher ready ({time.perf_counter() - t0:.1f} s)")
-    student = bench_models.water_student(seed=seed + 5)
+    if data is None:
+        data = list(teacher_snapshots(teacher, n_structures=n_structures, n_snapshots=n_snapshots, every=every,
+                                      seed=seed, log=log))
+    log(f"[student] {len(data)} snapshots labelled ({time.perf_counter() - t0:.1f} s)")
+    if select == "sensitivity":
+        ranking = rank_teacher_g3(teacher, [sn for sn, _, _ in data[:: max(1, len(data) // 48)]], seed=seed)
+        spec = bench_models.student_spec(r_cutoff, select_g3(ranking))
+        spec["ranking"] = {e: {p: [[sc, list(c)] for sc, c in v] for p, v in d.items()} for e, d in ranking.items()}
+    else:
+        spec = bench_models.student_spec(r_cutoff)
+    student = bench_models.student_from_spec(spec, seed=seed + 5)
     tr = StudentTrainer(student, precision="mixed", w_energy=1.0, w_force=w_force, lr=lr, seed=seed)
-    for snap, e, f in teacher_snapshots(teacher, n_structures=n_structures, n_snapshots=n_snapshots,
-                                        every=every, seed=seed, log=log):
+    for snap, e, f in data:
         tr.add(snap, e, f)
-    log(f"[student] {len(tr._raw)} snapshots labelled ({time.perf_counter() - t0:.1f} s)")
     tr.prepare(fit_scaler=True)
     n = len(tr.snapshots)
     perm = np.random.default_rng(seed).permutation(n)
@@ -311,7 +396,7 @@ def train_water_student(out_dir=ASSET_DIR, n_snapshots: int = 2000, n_structures
     before = tr.evaluate(val)
     log(f"[student] untrained: {before}")
     t1 = time.perf_counter()
-    hist = tr.train(epochs, batch=batch, train_idx=train,
+    hist = tr.train(epochs, batch=batch, train_idx=train, lr_schedule=cosine_schedule(lr, epochs),
                     log=lambda m: log(f"[student] {m}"))
     t_train = time.perf_counter() - t1
     after_val = tr.evaluate(val)
@@ -320,13 +405,16 @@ def train_water_student(out_dir=ASSET_DIR, n_snapshots: int = 2000, n_structures
     out = Path(out_dir)
     out.mkdir(parents=True, exist_ok=True)
     student.save(out)
+    (out / bench_models.STUDENT_SPEC).write_text(json.dumps(spec, indent=1))
     f_rms = math.sqrt(sum(float((sn.F_ref ** 2).sum()) for sn in tr.snapshots) /
                       (3 * sum(sn.n for sn in tr.snapshots))) * units.TO_MILLI_ELECTRON_VOLT
     metrics = {"snapshots": n, "validation": int(n_val), "epochs": epochs, "batch": batch, "lr": lr,
-               "w_force": w_force, "adam_steps": tr.step_count, "train_seconds": t_train,
+               "lr_schedule": f"{lr} for {epochs // 2} epochs, then cosine to 1e-5",
+               "w_force": w_force, "r_cutoff": r_cutoff, "select": select,
+               "adam_steps": tr.step_count, "train_seconds": t_train,
                "untrained_validation": before, "validation_metrics": after_val, "training_metrics": after_train,
                "teacher_force_rms_mev_per_bohr": f_rms,
-               "loss_history": [(int(a), float(b), float(c)) for a, b, c in hist[:: max(1, len(hist) // 50)]]}
+               "loss_history": [(int(a), float(b), float(c)) for a, b, c in hist[:: max(1, len(hist) // 60)]]}
     (out / "metrics.json").write_text(json.dumps(metrics, indent=1))
     return metrics
 
@@ -338,9 +426,11 @@ if __name__ == "__main__":
     ap.add_argument("--out", default=str(ASSET_DIR))
     ap.add_argument("--snapshots", type=int, default=2000)
     ap.add_argument("--structures", type=int, default=64)
-    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--epochs", type=int, default=600)
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--rc", type=float, default=12.0, help="student cutoff (Bohr)")
+    ap.add_argument("--select", default="sensitivity", choices=("sensitivity", "zeta1"))
     a = ap.parse_args()
     train_water_student(a.out, a.snapshots, a.structures, epochs=a.epochs, batch=a.batch, lr=a.lr,
-                        log=lambda m: print(m, flush=True))
+                        r_cutoff=a.rc, select=a.select, log=lambda m: print(m, flush=True))

@@ -18,6 +18,14 @@ failure rewinds to the last state that passed and hands the window to the
 judge's own model, which needs no judging.  A fallback at the judge's
 precision is the judge's model and is never judged.
 
+A pre-judge (``prejudge_margin``) watches the student's own inputs: after
+every student step the device counts descriptor rows outside the student's
+scaler range widened by the margin (pt_scaler_extrapolation, the check of
+descriptors/scaler.py:286-295), and every ``prejudge_every`` steps the host
+reads that count; rows out of range call the judge at once instead of at the
+next judge interval, so a student that leaves its training distribution is
+judged before it has driven long on extrapolated inputs.
+
 A sampled judge evaluates the teacher only where the sampled atoms' full
 forces need it: F_k = -sum_i dE_i/dr_k involves the centres i within the
 cutoff of k, so descriptors, MLP and adjoint run on the closed
@@ -57,6 +65,11 @@ class SurrogateStats:
     judge_rmse: list = field(default_factory=list)   # Ha/Bohr, one per judge
     judge_driver: list = field(default_factory=list)  # "student" or "fallback", one per judge
     list_overflows: int = 0        # judged windows redone because a gated list build overflowed
+    prejudge_triggers: int = 0     # judges called early by the extrapolation count
+    extrapolated_rows: int = 0     # student descriptor rows seen out of range (pre-judge)
+    # kept trajectory as windows (first step, end step, driver, judge RMSE in Ha/Bohr; None for
+    # steps of the judge's own model, which needs no judging)
+    windows: list = field(default_factory=list)
     seconds: float = 0.0
 
     @property
@@ -76,6 +89,8 @@ class SurrogateStats:
                 "reference_steps": self.reference_steps,
                 "discarded_steps": self.discarded_steps, "judges": self.judges, "failures": self.failures,
                 "fallback_failures": self.fallback_failures, "list_overflows": self.list_overflows,
+                "prejudge_triggers": self.prejudge_triggers, "extrapolated_rows": self.extrapolated_rows,
+                "student_kept_steps": sum(b - a for a, b, d, _ in self.windows if d == "student"),
                 "fallback_rate": self.fallback_rate, "teacher_share": self.teacher_share,
                 "judge_rmse_mev_per_bohr_median": float(np.median(r)) if r.size else None,
                 "judge_rmse_mev_per_bohr_max": float(np.max(r)) if r.size else None,
@@ -92,7 +107,8 @@ class SurrogateMD:
                  threshold: float = 5.0 / units.TO_MILLI_ELECTRON_VOLT, fallback_steps: int = 200,
                  skin: float = 1.0, thermostat: Optional[BrendsenThermostat] = None,
                  judge_atoms: Optional[int] = None, judge_seed: int = 0,
-                 probation_every: Optional[int] = None):
+                 probation_every: Optional[int] = None, graph: bool = False,
+                 prejudge_margin: Optional[float] = None, prejudge_every: int = 5):
         import torch
 
         if integrator not in ("reference", "physical"):
@@ -102,7 +118,8 @@ class SurrogateMD:
         if probation_every is not None and not 1 <= probation_every <= judge_every:
             raise ValueError("probation_every must be in [1, judge_every]")
         self.system = System.from_structure(structure, student, temperature=temperature, seed=seed,
-                                            force_mode="full", precision=student_precision, skin=skin)
+                                            force_mode="full", precision=student_precision, skin=skin,
+                                            graph=graph)
         md = self.system._engine()
         self.md = md
         self.f_student = md.ff
@@ -128,6 +145,14 @@ class SurrogateMD:
         self.rng = np.random.default_rng(judge_seed)
         self._flags = None
         self.stats = SurrogateStats()
+        self.prejudge_margin = prejudge_margin
+        self.prejudge_every = int(prejudge_every)
+        if prejudge_margin is not None:
+            from .descriptors import params_to_device
+
+            self._extra = torch.zeros(3, dtype=torch.int64, device=md.dev)
+            self._extra_params = {e: params_to_device(student.scalers_params[e], md.dev)
+                                  for e in student.elements if student.scalers_params.get(e) is not None}
         self.mode = "student"
         self.fallback_left = 0
         self.good = md.save_state()
@@ -194,6 +219,8 @@ class SurrogateMD:
     # ---- driving -----------------------------------------------------------------
     def _to_student(self) -> None:
         md = self.md
+        if self.mode == "reference" and self.stats.steps > self.good_step:
+            self.stats.windows.append((self.good_step, self.stats.steps, "reference", None))
         md.use(self.f_student)
         md.compute(md.force, "full")
         self.good = md.save_state()
@@ -220,6 +247,8 @@ class SurrogateMD:
         """Judge the driver at the current positions; on a pass the state becomes the rewind point."""
         st = self.stats
         rmse = self.judge()
+        if self.prejudge_margin is not None:
+            self._prejudge_due()   # a judge covers every step before it
         # the judge synchronises: check the gated list builds of the window with it
         if not self.md.lists_ok():
             self.md.grow_lists()
@@ -229,6 +258,7 @@ class SurrogateMD:
         st.judge_rmse.append(rmse)
         st.judge_driver.append(self.mode)
         if rmse <= self.threshold:
+            st.windows.append((self.good_step, st.steps, self.mode, rmse))
             self.good = self.md.save_state()
             self.good_step = st.steps
             return True
@@ -237,8 +267,10 @@ class SurrogateMD:
             st.fallback_failures += 1
         return False
 
-    def run(self, n_steps: int) -> SurrogateStats:
-        """Advance the kept trajectory by ``n_steps`` steps."""
+    def run(self, n_steps: int, judge_tail: bool = False) -> SurrogateStats:
+        """Advance the kept trajectory by ``n_steps`` steps.  judge_tail: the steps after the
+        last judge are judged too (and redone by the fallback if they fail), so that every kept
+        step lies in a judged window (or one of the judge's own model)."""
         import torch
 
         md = self.md
@@ -248,14 +280,43 @@ class SurrogateMD:
         while True:
             self._advance(target)
             # steps after the last judge: their list builds are checked before they count
-            if md.lists_ok():
-                break
-            md.grow_lists()
-            st.list_overflows += 1
-            self._to_fallback()
+            if not md.lists_ok():
+                md.grow_lists()
+                st.list_overflows += 1
+                self._to_fallback()
+                continue
+            if judge_tail and st.steps > self.good_step:
+                if self.mode == "reference":
+                    st.windows.append((self.good_step, st.steps, "reference", None))
+                    self.good = md.save_state()
+                    self.good_step = st.steps
+                elif not self._judge_step():
+                    self._to_fallback(reference=self.mode == "fallback")
+                    continue
+            break
         torch.cuda.synchronize()
         st.seconds += time.perf_counter() - t0
         return st
+
+    def _count_extrapolation(self) -> None:
+        """Add the student's out-of-range descriptor rows of this step to the device counter."""
+        from .descriptors import extrapolation_counts
+
+        ff = self.f_student
+        nl = self.md.nl
+        for e, b, cnt in ff._blocks(nl):
+            p = self._extra_params.get(e)
+            if p is not None:
+                n_sf = ff.pot.atomic_potentials[e].model_input_size
+                extrapolation_counts(ff.G[b:b + cnt, :n_sf], p, self._extra, self.prejudge_margin)
+
+    def _prejudge_due(self) -> bool:
+        """Whether the student extrapolated since the last read (a host read)."""
+        rows = int(self._extra[0].item())
+        if rows:
+            self.stats.extrapolated_rows += rows
+            self._extra.zero_()
+        return rows > 0
 
     def _advance(self, target: int) -> None:
         md = self.md
@@ -266,7 +327,14 @@ class SurrogateMD:
             if self.mode == "student":
                 st.student_steps += 1
                 every = self.probation_every if self.on_probation else self.judge_every
-                if st.steps - self.good_step >= every:
+                early = False
+                if self.prejudge_margin is not None:
+                    self._count_extrapolation()
+                    if (st.steps - self.good_step) % self.prejudge_every == 0 and \
+                            st.steps - self.good_step < every and self._prejudge_due():
+                        early = True
+                        st.prejudge_triggers += 1
+                if early or st.steps - self.good_step >= every:
                     passed = self._judge_step()
                     self.on_probation = not passed
                     if not passed:

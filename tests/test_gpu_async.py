@@ -145,12 +145,12 @@ def test_gated_permutation(gate_value):
         assert torch.equal(buf, want)
 
 
-def _trajectory(s, pot, precision, sync_free, steps=60, row_bound=None, thermostat=True):
+def _trajectory(s, pot, precision, sync_free, steps=60, row_bound=None, thermostat=True, skin=1.0, graph=False):
     from pantea_amd.simulation import BrendsenThermostat, MDSimulator, System
     from pantea_amd.units import units
 
     sysm = System.from_structure(s, pot, temperature=300.0, seed=2023, force_mode="full", precision=precision,
-                                 skin=1.0, sync_free=sync_free)
+                                 skin=skin, sync_free=sync_free, graph=graph)
     md = sysm._engine()
     if row_bound is not None:
         md.row_bound = md.nl.max_row + row_bound
@@ -183,10 +183,26 @@ def test_row_bound_overflow_is_redone():
 
     s = _water(128, 29.56)
     pot = water_nnp(s, hidden=(25, 25), seed=2023)
-    x0, v0, _, _ = _trajectory(s, pot, "float64", False, steps=30)
-    # a bound a few entries below the longest row: the first rebuild exceeds it
-    x1, v1, _, redos = _trajectory(s, pot, "float64", True, steps=30, row_bound=-8)
+    x0, v0, _, _ = _trajectory(s, pot, "float64", False, steps=30, skin=0.0)
+    # without a skin every step rebuilds; a bound a few entries below the longest row of
+    # the start is exceeded by the first rebuild
+    x1, v1, _, redos = _trajectory(s, pot, "float64", True, steps=30, row_bound=-8, skin=0.0)
     assert redos >= 1
     assert np.isfinite(x1).all()
+    np.testing.assert_array_equal(x1, x0)
+    np.testing.assert_array_equal(v1, v0)
+
+
+@pytest.mark.parametrize("precision", ["float64", "mixed"])
+def test_graph_replayed_md_equals_synchronous_md(precision):
+    """The gated step captured once into a HIP graph and replayed (recaptured after each
+    checkpoint restore and buffer change) gives the synchronous trajectory bit for bit."""
+    from pantea_amd.bench_models import water_nnp
+
+    s = _water(128, 29.56)
+    pot = water_nnp(s, hidden=(25, 25), seed=2023)
+    x0, v0, b0, _ = _trajectory(s, pot, precision, False)
+    x1, v1, b1, _ = _trajectory(s, pot, precision, True, graph=True)
+    assert b0 == b1
     np.testing.assert_array_equal(x1, x0)
     np.testing.assert_array_equal(v1, v0)

@@ -405,3 +405,45 @@ def test_sampled_judge_forces_equal_full_evaluation():
     st = sur.run(30)   # an untrained student: judged once at step 10, rejected, teacher to the end
     assert st.judges == st.failures == 1 and st.fallback_steps == 30
     assert all(math.isfinite(r) for r in st.judge_rmse)
+
+
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-12), ("mixed", 1e-5)])
+def test_transposed_edge_forces_equal_reverse_index_gather(precision, tol):
+    """The MD force field writes edge gradients at the reverse edges' slots
+    (pt_acsf_adjoint_transposed) and gathers rows contiguously; the potential's
+    evaluation uses the reverse-index gather.  Same -grad E (the FP32 edges round
+    the self term differently: it is summed in FP64 before rounding)."""
+    from pantea_amd.simulation import System
+
+    s, pot = _cfg0()
+    sysm = System.from_structure(s, pot, temperature=300.0, seed=1, force_mode="full", precision=precision,
+                                 skin=1.0)
+    f_md = sysm.forces.cpu().numpy()
+    _, f_pot = pot.energy_and_forces(s, mode="full", precision=precision)
+    f_pot = f_pot.cpu().numpy()
+    assert np.max(np.abs(f_md - f_pot)) <= tol * np.max(np.abs(f_pot))
+
+
+def test_prejudge_calls_the_judge_when_the_student_extrapolates():
+    """The extrapolation count of the student's descriptors (pt_scaler_extrapolation) calls
+    the judge early; with a margin no descriptor reaches, the judge keeps its interval."""
+    from pantea_amd.bench_models import make_nnp, student_acsf
+    from pantea_amd.simulation import BrendsenThermostat
+    from pantea_amd.surrogate import SurrogateMD
+    from pantea_amd.units import units
+
+    s, teacher = _cfg0()
+    student = make_nnp(student_acsf, (16, 16), seed=99, precision="mixed")
+    student.fit_scaler([s])   # the start structure only: MD soon leaves its range
+    runs = {}
+    for margin in (0.0, 1e6):
+        sur = SurrogateMD(teacher, student, s, seed=5, judge_every=20, threshold=math.inf, fallback_steps=20,
+                          skin=1.0, thermostat=BrendsenThermostat(300.0, 0.5 * units.FROM_FEMTO_SECOND),
+                          prejudge_margin=margin, prejudge_every=5)
+        runs[margin] = sur.run(40, judge_tail=True).as_dict()
+        print(margin, runs[margin])
+    assert runs[0.0]["prejudge_triggers"] > 0 and runs[0.0]["extrapolated_rows"] > 0
+    assert runs[0.0]["judges"] > runs[1e6]["judges"]
+    assert runs[1e6]["prejudge_triggers"] == 0 and runs[1e6]["judges"] == 2
+    for r in runs.values():   # the threshold passes every judge: the student drives all
+        assert r["failures"] == 0 and r["student_kept_steps"] == 40 and r["steps"] == 40

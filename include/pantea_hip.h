@@ -320,6 +320,43 @@ int pt_adam_step(int n, double *params, const double *grad, double *m, double *v
                  pt_stream_t stream);
 
 /* ------------------------------------------------------------------------
+ * (f) Global extended Kalman filter trainer
+ * (potentials/nnp/kalman_filter.py:42-335).
+ *
+ * pt_mlp_jacobian: per-row Jacobian of the reference's self-term forces
+ * (the jacrev of _compute_forces_error, kalman_filter.py:305-335).  Row
+ * (atom a, direction d) of J (row-major, leading dimension ldj, columns in
+ * the layout of the unpacked params) is
+ *     sign * d/dtheta [ grad_x' y(x'_a) . (T_d[a] * mul) ],
+ * with T_d the centre-only descriptor tangent along axis d (pt_acsf_tangent
+ * with center_only and delta = e_d): sign = -1 gives dF_self[a, d]/dtheta.
+ * G: (device) n_atoms x ldg FP64; T: n_dir blocks of n_atoms x ldg FP64,
+ * t_dir doubles apart; J: (device, out) n_atoms * n_dir rows; only the
+ * columns of this element's parameters are written.  FP64 only.
+ * ---------------------------------------------------------------------- */
+int pt_mlp_jacobian(int n_atoms, int n_dir, const double *G, const double *T, long long t_dir, int ldg,
+                    const double *scale, const pt_mlp_desc *desc, const void *packed, double sign,
+                    double *J, long long ldj, pt_stream_t stream);
+/* xi[i] = (ref[i] - pred[i]) * scale (ref == NULL: ref_scalar for every i) and
+ * loss[0] += sum_i xi[i]^2: the energy and force errors of
+ * kalman_filter.py:276-303 and the loss of :243, :268. */
+int pt_kalman_residual(int n, const double *pred, const double *ref, double ref_scalar, double scale,
+                       double *xi, double *loss, pt_stream_t stream);
+/* One update of KalmanFilter.update_state_vector (kalman_filter.py:196-233)
+ * with the schedules applied by the caller:
+ *   X = P H, A = H^T X + r I, K = X A^-1, P <- c (P - K X^T) + q I, W <- W + K xi
+ * r = 1/eta (type 0) or lambda (type 1), c = 1 (type 0) or 1/lambda (type 1).
+ * P: (device, in/out) n_w x n_w FP64; H: (device) m columns of n_w (leading
+ * dimension ldh), column o = gradient of observation o; xi: m; W: n_w.
+ * A is factorised by Cholesky (rocSOLVER dpotrf/dpotrs) instead of being
+ * inverted; info (device int) receives dpotrf's status (0 = success).
+ * ws: pt_kalman_workspace_bytes(n_w, m) bytes of device memory. */
+size_t pt_kalman_workspace_bytes(int n_w, int m);
+int pt_kalman_update(int n_w, int m, double *P, const double *H, long long ldh, const double *xi,
+                     double *W, double r, double c, double q, void *ws, size_t ws_bytes, int *info,
+                     pt_stream_t stream);
+
+/* ------------------------------------------------------------------------
  * Scaler statistics and extrapolation counter on the device
  * (descriptors/scaler.py:84-142, 249-295).  params (device) holds 4 x n_cols
  * FP64: mean | sigma | minval | maxval (the ScalerParams fields).  G: n_rows

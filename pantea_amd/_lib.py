@@ -139,6 +139,11 @@ _SIGS = {
     "pt_train_seeds": (_c_int, [_c_int, _vp, _vp, _vp, _c_dbl, _c_dbl, _c_dbl, _vp, _vp, _vp, _vp]),
     "pt_adam_step": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _c_dbl,
                               _c_int, _vp]),
+    "pt_mlp_jacobian": (_c_int, [_c_int, _c_int, _vp, _vp, _c_ll, _c_int, _vp, _vp, _vp, _c_dbl, _vp, _c_ll, _vp]),
+    "pt_kalman_residual": (_c_int, [_c_int, _vp, _vp, _c_dbl, _c_dbl, _vp, _vp, _vp]),
+    "pt_kalman_workspace_bytes": (_c_size, [_c_int, _c_int]),
+    "pt_kalman_update": (_c_int, [_c_int, _c_int, _vp, _vp, _c_ll, _vp, _vp, _c_dbl, _c_dbl, _c_dbl, _vp, _c_size,
+                                  _vp, _vp]),
 }
 
 #: Every symbol the header declares; the CPU test suite checks each is exported.

@@ -35,7 +35,10 @@ HIPCC_FLAGS = [
     f"-I{CSRC}",
 ]
 
-SOURCES = ["md.hip", "nbr.hip", "acsf.hip", "mlp.hip", "scaler.hip"]
+SOURCES = ["md.hip", "nbr.hip", "acsf.hip", "mlp.hip", "scaler.hip", "kalman.hip"]
+
+# the Kalman filter update calls rocBLAS (GEMM, GEMV) and rocSOLVER (Cholesky)
+LINK_FLAGS = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
 
 
 def hipcc() -> str:
@@ -48,7 +51,7 @@ def hipcc() -> str:
 def build(verbose: bool = False) -> Path:
     srcs = [str(CSRC / s) for s in SOURCES]
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, *srcs, "-o", str(tmp)]
+    cmd = [hipcc(), *HIPCC_FLAGS, *srcs, *LINK_FLAGS, "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

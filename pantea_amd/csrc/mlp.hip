@@ -753,6 +753,214 @@ int launch_train(TrainArgs a, hipStream_t st) {
     return launch_check("k_mlp_train");
 }
 
+// ---- per-row Jacobian of the input-gradient projection (FP64) -----------------
+//
+// Row (atom r, direction d) of J is  sign * d(ydot_d)/dtheta,  with
+// ydot_d = grad_x' MLP . (T_d * mul) the MLP output's derivative along the
+// scaled tangent of direction d.  The forward-over-reverse sweep is the one of
+// k_mlp_train (primal rows 0..15 once per tile, tangent rows 16..31 once per
+// direction), but nothing is summed over atoms: a dense layer's block of a
+// row is the outer product  a[r] (x) zbar[r] + adot[r] (x) zdotbar[r]  and a
+// bias block is zbar[r], written straight to HBM with consecutive lanes on
+// consecutive parameters of one row.
+struct JacArgs {
+    MlpShape s;
+    int n_atoms;
+    const double* G;
+    const double* T;      // T[d * t_dir + atom * ldg + k]
+    long long t_dir;
+    int n_dir;
+    int ldg;
+    const double* scale;
+    double scale_add;
+    const char* packed;
+    double sign;
+    double* J;            // row (atom * n_dir + d), unpacked parameter layout
+    long long ldj;
+    int stride;
+};
+
+__global__ __launch_bounds__(256) void k_mlp_jacobian(JacArgs a) {
+    typedef double T;
+    constexpr int MODE = PT_MLP_F64;
+    extern __shared__ __align__(16) char smem[];
+    const MlpShape& s = a.s;
+    const int H = s.n_hidden;
+    const int l = wave_lane();
+    const int wv = threadIdx.x >> 6;
+    const int wpb = blockDim.x >> 6;
+    const int stride = a.stride;
+    const size_t tile = (size_t)32 * stride;
+    const size_t per_wave = (size_t)(2 * H + 3) * tile + 64;
+    T* base = reinterpret_cast<T*>(smem) + (size_t)wv * per_wave;
+    T* Abuf[kMaxHidden + 1];
+    T* Zbuf[kMaxHidden];
+    for (int h = 0; h <= H; ++h) Abuf[h] = base + h * tile;
+    for (int h = 0; h < H; ++h) Zbuf[h] = base + (H + 1 + h) * tile;
+    T* P[2] = {base + (2 * H + 1) * tile, base + (2 * H + 2) * tile};
+    T* outv = base + (2 * H + 3) * tile;  // [0,16): z_L, [32,48): zbar_L, [48,64): zdotbar_L
+    const int row = l & 15, grp = l >> 4;
+    const T* wout = (const T*)(a.packed + s.wout64);
+    const T bout = ((const double*)(a.packed + s.bout64))[0];
+    const int L = H;
+    const int fan_L = H > 0 ? s.w[H - 1] : s.in_dim;
+    const int ntiles = (a.n_atoms + 15) / 16;
+    // one work item = (16-atom tile, direction): the primal sweep is redone per direction
+    for (int it = blockIdx.x * wpb + wv; it < ntiles * a.n_dir; it += gridDim.x * wpb) {
+        const int a0 = (it / a.n_dir) * 16;
+        const int d = it % a.n_dir;
+        const int nrows = min(16, a.n_atoms - a0);
+        for (int e = l; e < 16 * s.in_p; e += kWave) {
+            const int r = e / s.in_p, k = e % s.in_p;
+            T v = T(0);
+            if (r < nrows && k < s.in_dim)
+                v = (a.G[(size_t)(a0 + r) * a.ldg + k] - a.scale[k]) * a.scale[s.in_dim + k] + a.scale_add;
+            Abuf[0][r * stride + k] = v;
+        }
+        wave_fence();
+        // primal forward, rows 0..15 (kept for every direction)
+        int kin = s.in_p;
+        for (int h = 0; h < H; ++h) {
+            const int wp = s.wp[h];
+            for (int n0 = 0; n0 < wp; n0 += 16) {
+                T az[4] = {T(0), T(0), T(0), T(0)};
+                gemm_tile<MODE>(Abuf[h], stride, kin, a.packed, s.W64[h], 0, wp, false, n0, az);
+                const int col = n0 + row;
+                const T bias = ((const double*)(a.packed + s.b64[h]))[col];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int r = crow<MODE>(grp, i);
+                    T y, dy;
+                    const T z = az[i] + bias;
+                    activate<T>(s.act[h], z, y, dy);
+                    const bool pad = col >= s.w[h];
+                    Zbuf[h][r * stride + col] = pad ? T(0) : z;
+                    Abuf[h + 1][r * stride + col] = pad ? T(0) : y;
+                }
+            }
+            wave_fence();
+            kin = wp;
+        }
+        {
+            const int r = l >> 2, part = l & 3;
+            T z = T(0);
+            for (int k = part; k < kin; k += 4) z += Abuf[H][r * stride + k] * wout[k];
+            z += __shfl_xor(z, 1, kWave);
+            z += __shfl_xor(z, 2, kWave);
+            if (part == 0) outv[r] = z + bout;
+        }
+        wave_fence();
+        {
+            // tangent input, rows 16..31
+            for (int e = l; e < 16 * s.in_p; e += kWave) {
+                const int r = e / s.in_p, k = e % s.in_p;
+                T t = T(0);
+                if (r < nrows && k < s.in_dim)
+                    t = a.T[(size_t)d * a.t_dir + (size_t)(a0 + r) * a.ldg + k] * a.scale[s.in_dim + k];
+                Abuf[0][(16 + r) * stride + k] = t;
+            }
+            wave_fence();
+            kin = s.in_p;
+            for (int h = 0; h < H; ++h) {
+                const int wp = s.wp[h];
+                for (int n0 = 0; n0 < wp; n0 += 16) {
+                    T at[4] = {T(0), T(0), T(0), T(0)};
+                    gemm_tile<MODE>(Abuf[h] + 16 * stride, stride, kin, a.packed, s.W64[h], 0, wp, false, n0, at);
+                    const int col = n0 + row;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int r = crow<MODE>(grp, i);
+                        T y, dy;
+                        activate<T>(s.act[h], Zbuf[h][r * stride + col], y, dy);
+                        const bool pad = col >= s.w[h];
+                        Zbuf[h][(16 + r) * stride + col] = pad ? T(0) : at[i];
+                        Abuf[h + 1][(16 + r) * stride + col] = pad ? T(0) : dy * at[i];
+                    }
+                }
+                wave_fence();
+                kin = wp;
+            }
+            // output layer: seeds of the projection's reverse sweep
+            {
+                const int r = l >> 2, part = l & 3;
+                T zd = T(0);
+                for (int k = part; k < kin; k += 4) zd += Abuf[H][(16 + r) * stride + k] * wout[k];
+                zd += __shfl_xor(zd, 1, kWave);
+                zd += __shfl_xor(zd, 2, kWave);
+                T y, dy, ddy;
+                activate2<T>(s.act_out, outv[r], y, dy, ddy);
+                const T gt = r < nrows ? (T)a.sign : T(0);
+                if (part == 0) {
+                    outv[32 + r] = gt * ddy * zd;  // zbar_L
+                    outv[48 + r] = gt * dy;        // zdotbar_L
+                }
+            }
+            wave_fence();
+            // rows of J: output layer
+            for (int e = l; e < 16 * fan_L; e += kWave) {
+                const int r = e / fan_L, k = e % fan_L;
+                if (r < nrows)
+                    a.J[(size_t)((a0 + r) * a.n_dir + d) * a.ldj + s.pW[L] + k] =
+                        Abuf[H][r * stride + k] * outv[32 + r] + Abuf[H][(16 + r) * stride + k] * outv[48 + r];
+            }
+            if (l < nrows) a.J[(size_t)((a0 + l) * a.n_dir + d) * a.ldj + s.pb[L]] = outv[32 + l];
+            for (int e = l; e < 16 * kin; e += kWave) {
+                const int r = e / kin, k = e % kin;
+                P[0][r * stride + k] = outv[32 + r] * wout[k];
+                P[0][(16 + r) * stride + k] = outv[48 + r] * wout[k];
+            }
+            wave_fence();
+            int cur = 0;
+            for (int h = H - 1; h >= 0; --h) {
+                const int wp = s.wp[h];
+                const int kin_h = h > 0 ? s.wp[h - 1] : s.in_p;
+                for (int e = l; e < 16 * wp; e += kWave) {
+                    const int r = e / wp, k = e % wp;
+                    const T z = Zbuf[h][r * stride + k], zd = Zbuf[h][(16 + r) * stride + k];
+                    T y, dy, ddy;
+                    activate2<T>(s.act[h], z, y, dy, ddy);
+                    const T ab = P[cur][r * stride + k], adb = P[cur][(16 + r) * stride + k];
+                    const bool pad = k >= s.w[h];
+                    P[cur][r * stride + k] = pad ? T(0) : ab * dy + adb * ddy * zd;
+                    P[cur][(16 + r) * stride + k] = pad ? T(0) : adb * dy;
+                }
+                wave_fence();
+                // rows of J: W_h[in][out] and b_h[out]
+                const int fan_in = h > 0 ? s.w[h - 1] : s.in_dim;
+                const int wo = s.w[h];
+                const int nW = fan_in * wo;
+                for (int e = l; e < 16 * nW; e += kWave) {
+                    const int r = e / nW, q = e % nW;
+                    const int in = q / wo, out = q % wo;
+                    if (r < nrows)
+                        a.J[(size_t)((a0 + r) * a.n_dir + d) * a.ldj + s.pW[h] + q] =
+                            Abuf[h][r * stride + in] * P[cur][r * stride + out] +
+                            Abuf[h][(16 + r) * stride + in] * P[cur][(16 + r) * stride + out];
+                }
+                for (int e = l; e < 16 * wo; e += kWave) {
+                    const int r = e / wo, out = e % wo;
+                    if (r < nrows) a.J[(size_t)((a0 + r) * a.n_dir + d) * a.ldj + s.pb[h] + out] = P[cur][r * stride + out];
+                }
+                if (h > 0) {
+                    for (int n0 = 0; n0 < kin_h; n0 += 16) {
+                        T c0[4] = {T(0), T(0), T(0), T(0)}, c1[4] = {T(0), T(0), T(0), T(0)};
+                        gemm_tile<MODE>(P[cur], stride, wp, a.packed, s.W64[h], 0, wp, true, n0, c0);
+                        gemm_tile<MODE>(P[cur] + 16 * stride, stride, wp, a.packed, s.W64[h], 0, wp, true, n0, c1);
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const int r = crow<MODE>(grp, i);
+                            P[cur ^ 1][r * stride + n0 + row] = c0[i];
+                            P[cur ^ 1][(16 + r) * stride + n0 + row] = c1[i];
+                        }
+                    }
+                    cur ^= 1;
+                }
+                wave_fence();
+            }
+        }
+    }
+}
+
 // ---- loss seeds and Adam --------------------------------------------------------
 
 __global__ void k_train_seeds(int n, const double* __restrict__ f, const double* __restrict__ fref,
@@ -901,6 +1109,46 @@ extern "C" int pt_mlp_train_grad(int mode, int n_atoms, const void* G, const voi
     }
     set_error("pt_mlp_train_grad: unknown mode %d", mode);
     return PT_ERR_ARG;
+}
+
+extern "C" int pt_mlp_jacobian(int n_atoms, int n_dir, const double* G, const double* T, long long t_dir, int ldg,
+                               const double* scale, const pt_mlp_desc* desc, const void* packed, double sign,
+                               double* J, long long ldj, pt_stream_t stream) {
+    PT_REQUIRE(n_atoms >= 0 && n_dir >= 1, "pt_mlp_jacobian: n_atoms < 0 or n_dir < 1");
+    if (n_atoms == 0) return PT_OK;
+    PT_REQUIRE(G && T && scale && desc && packed && J, "pt_mlp_jacobian: null argument");
+    JacArgs a{};
+    if (int rc = make_shape(desc, a.s)) return rc;
+    PT_REQUIRE(ldg >= desc->in_dim, "pt_mlp_jacobian: ldg < in_dim");
+    PT_REQUIRE(t_dir >= (long long)(n_atoms - 1) * ldg + desc->in_dim || n_dir == 1,
+               "pt_mlp_jacobian: tangent directions overlap");
+    const long long np = (long long)(a.s.pb[a.s.n_hidden] + 1);
+    PT_REQUIRE(ldj >= np, "pt_mlp_jacobian: ldj below the parameter count");
+    PT_REQUIRE((long long)n_atoms * n_dir < (1LL << 31), "pt_mlp_jacobian: too many rows");
+    a.n_atoms = n_atoms;
+    a.n_dir = n_dir;
+    a.G = G;
+    a.T = T;
+    a.t_dir = t_dir;
+    a.ldg = ldg;
+    a.scale = scale;
+    a.scale_add = desc->scale_add;
+    a.packed = (const char*)packed;
+    a.sign = sign;
+    a.J = J;
+    a.ldj = ldj;
+    a.stride = a.s.wmax + 1;
+    const size_t per_wave = sizeof(double) * ((size_t)(2 * a.s.n_hidden + 3) * 32 * a.stride + 64);
+    const size_t budget = lds_limit();
+    if (per_wave > budget) {
+        set_error("pt_mlp_jacobian: %zu bytes of LDS per wavefront exceed the device limit", per_wave);
+        return PT_ERR_UNSUPPORTED;
+    }
+    const int wpb = (int)std::min<size_t>(4, budget / per_wave);
+    const long long items = (long long)((n_atoms + 15) / 16) * n_dir;
+    const int blocks = (int)std::max<long long>(1, std::min<long long>((items + wpb - 1) / wpb, 8192));
+    hipLaunchKernelGGL(k_mlp_jacobian, dim3(blocks), dim3(64 * wpb), wpb * per_wave, (hipStream_t)stream, a);
+    return launch_check("k_mlp_jacobian");
 }
 
 extern "C" int pt_train_seeds(int n, const double* force, const double* force_ref, const double* energy,

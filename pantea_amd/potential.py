@@ -42,6 +42,10 @@ _ACT_BY_CODE = {"l": "identity", "t": "tanh", "s": "logistic", "p": "softplus", 
 _SCALER_BY_KEY = {"center_symmetry_functions": "center", "scale_symmetry_functions": "scale",
                   "scale_center_symmetry_functions": "scale_center",
                   "scale_center_symmetry_functions_sigma": "scale_center_sigma"}
+_UPDATER_BY_CODE = {"0": "gradient_descent", "1": "kalman_filter"}
+_FLOAT_KEYS = ("force_weight", "short_force_fraction", "short_energy_fraction", "kalman_epsilon", "kalman_q0",
+               "kalman_qtau", "kalman_qmin", "kalman_eta", "kalman_etatau", "kalman_etamax", "kalman_lambda_short",
+               "kalman_neu_short")
 
 PRECISION_MODES = {
     # name: (descriptor precision, MLP mode)
@@ -82,6 +86,22 @@ class NNPSettings:
     scale_type: str = "center"
     scale_min_short: float = 0.0
     scale_max_short: float = 1.0
+    # trainer keywords (settings.py:106-131)
+    epochs: int = 1
+    updater_type: str = "gradient_descent"
+    force_weight: float = 1.0
+    short_force_fraction: float = 0.1
+    short_energy_fraction: float = 1.0
+    kalman_type: int = 0
+    kalman_epsilon: float = 0.01
+    kalman_q0: float = 0.01
+    kalman_qtau: float = 2.302
+    kalman_qmin: float = 1.0e-6
+    kalman_eta: float = 0.01
+    kalman_etatau: float = 2.302
+    kalman_etamax: float = 1.0
+    kalman_lambda_short: float = 0.96
+    kalman_neu_short: float = 0.9995
     extra: Dict[str, Any] = field(default_factory=dict)
 
     @classmethod
@@ -136,8 +156,12 @@ class NNPSettings:
                 kw["global_nodes_short"] = [int(t) for t in args]
             elif key == "global_activation_short":
                 kw["global_activation_short"] = [_ACT_BY_CODE[t] for t in args]
-            elif key in ("weights_min", "weights_max", "scale_min_short", "scale_max_short"):
+            elif key in ("weights_min", "weights_max", "scale_min_short", "scale_max_short") or key in _FLOAT_KEYS:
                 kw[key] = float(args[0])
+            elif key in ("epochs", "kalman_type"):
+                kw[key] = int(args[0])
+            elif key == "updater_type":
+                kw[key] = _UPDATER_BY_CODE[args[0]]
             elif key == "cutoff_type":
                 kw["cutoff_type"] = _CUTOFF_BY_CODE[args[0]]
             elif key in _SCALER_BY_KEY:

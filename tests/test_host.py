@@ -159,6 +159,29 @@ def test_settings_from_nn(tmp_path):
     r, a = s.symfunction_short
     assert (r.eta, r.r_shift, r.r_cutoff) == (0.01, 0.0, 12.0)
     assert (a.eta, a.lambda0, a.zeta, a.r_cutoff) == (0.02, -1.0, 4.0, 12.0)
+    # trainer keywords keep the reference's defaults (settings.py:106-131)
+    assert (s.epochs, s.updater_type, s.kalman_type, s.kalman_epsilon, s.kalman_lambda_short) == \
+        (1, "gradient_descent", 0, 0.01, 0.96)
+
+
+def test_settings_trainer_keywords(tmp_path):
+    from pantea_amd.potential import NNPSettings
+
+    p = tmp_path / "input.nn"
+    p.write_text(
+        "elements H O\nglobal_nodes_short 5 5\nglobal_activation_short t t l\n"
+        "epochs 7\nupdater_type 1\nforce_weight 3.5\nshort_force_fraction 0.25\nshort_energy_fraction 0.75\n"
+        "kalman_type 1\nkalman_epsilon 0.02\nkalman_q0 0.001\nkalman_qtau 1.5\nkalman_qmin 1e-7\n"
+        "kalman_eta 0.05\nkalman_etatau 2.0\nkalman_etamax 0.5\nkalman_lambda_short 0.98\n"
+        "kalman_neu_short 0.999\n")
+    s = NNPSettings.from_file(p)
+    assert (s.epochs, s.updater_type, s.force_weight, s.short_force_fraction, s.short_energy_fraction) == \
+        (7, "kalman_filter", 3.5, 0.25, 0.75)
+    assert (s.kalman_type, s.kalman_epsilon, s.kalman_q0, s.kalman_qtau, s.kalman_qmin, s.kalman_eta,
+            s.kalman_etatau, s.kalman_etamax, s.kalman_lambda_short, s.kalman_neu_short) == \
+        (1, 0.02, 0.001, 1.5, 1e-7, 0.05, 2.0, 0.5, 0.98, 0.999)
+    j = NNPSettings.from_file(GOLDEN / "h2o.json")
+    assert (j.epochs, j.updater_type, j.force_weight) == (20, "gradient_descent", 10.0)
 
 
 def test_runner_reader():

@@ -307,7 +307,8 @@ def run_surrogate(teacher, student, s, steps, warmup, args, instrument=False, th
     timers, sur.md.timers = sur.md.timers, None
     after = sur.stats.as_dict()
     delta = {k: after[k] - before[k] for k in ("steps", "student_steps", "fallback_steps", "reference_steps",
-                                                "discarded_steps", "judges", "failures", "fallback_failures")}
+                                                "discarded_steps", "judges", "failures", "fallback_failures",
+                                                "student_kept_steps")}
     r = np.asarray(sur.stats.judge_rmse[n_rmse:]) * units.TO_MILLI_ELECTRON_VOLT
     who = np.asarray(sur.stats.judge_driver[n_rmse:])
     delta["fallback_precision"] = args.fallback_precision
@@ -523,7 +524,7 @@ def main() -> int:
     judged = f"{args.judge_atoms} sampled atoms" if args.judge_atoms else "all atoms"
     fb = ("mixed-precision teacher steps (judged the same way; failing -> FP64 teacher)"
           if args.fallback_precision == "mixed" else "FP64 teacher steps")
-    drove = bool(sur and sur["student_steps"] > 0)
+    drove = bool(sur and sur["student_kept_steps"] > 0)   # steps of the student that no rewind discarded
     if sur is None:
         head = "mixed-precision teacher (no trained student found)"
     else:
@@ -532,7 +533,7 @@ def main() -> int:
                 f"{args.probation_every}) on {judged}, > {args.threshold_mev} meV/Bohr -> rewind + "
                 f"{args.fallback_steps} {fb}")
         if not drove:
-            head += ("; in the timed window the student drove no step (it fails the judge), so the value is the "
+            head += ("; in the timed window no step of the student was kept (it fails the judge), so the value is the "
                      "judged mixed-precision teacher's rate: the surrogate's own rate at this threshold is "
                      "UNMEASURED (see surrogate_at_student_accuracy)")
     out = {

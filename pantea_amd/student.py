@@ -353,7 +353,7 @@ def cosine_schedule(lr: float, epochs: int, hold: float = 0.5, lr_min: float = 1
 
 def train_water_student(out_dir=ASSET_DIR, n_snapshots: int = 2000, n_structures: int = 64, every: int = 10,
                         epochs: int = 600, batch: int = 32, lr: float = 1e-3, w_force: float = 10.0,
-                        seed: int = 2023, r_cutoff: float = 12.0, select: str = "sensitivity",
+                        seed: int = 2023, r_cutoff: float = 8.0, select: str = "sensitivity",
                         data=None, log=print) -> dict:
     """Generate teacher data, train the water student on the GPU and save it to ``out_dir``.
 
@@ -429,7 +429,7 @@ if __name__ == "__main__":
     ap.add_argument("--epochs", type=int, default=600)
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--rc", type=float, default=12.0, help="student cutoff (Bohr)")
+    ap.add_argument("--rc", type=float, default=8.0, help="student cutoff (Bohr)")
     ap.add_argument("--select", default="sensitivity", choices=("sensitivity", "zeta1"))
     a = ap.parse_args()
     train_water_student(a.out, a.snapshots, a.structures, epochs=a.epochs, batch=a.batch, lr=a.lr,

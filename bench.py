@@ -571,6 +571,7 @@ def main() -> int:
                          "surrogate.*_judge_rmse_*: the driver (student / mixed fallback) vs the FP64 "
                          "teacher at its judges",
         "force_rms_mev_per_bohr": f_rms,
+        "threshold_over_force_rms": args.threshold_mev / f_rms,
         "phase_ms_per_step": {k: v[0] / phase_steps for k, v in phases.items()},
         "phase_note": f"HIP events of instrumented repeats ({phase_steps} headline steps, {phase_steps_m} mixed "
                       "teacher steps, launched from the host); the timed runs carry no events",

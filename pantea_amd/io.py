@@ -11,40 +11,44 @@ from pathlib import Path
 from typing import Dict, Iterator, List, Optional
 
 
-def _tokenize(line: str):
-    tok = line.split()
-    if not tok:
-        return None, []
-    return tok[0].lower(), tok[1:]
+# The RuNNer input.data format, one record per line between "begin" and "end":
+#   atom x y z element charge energy fx fy fz | lattice ax ay az | energy E | charge Q | comment text
+# Per record keyword: (output field, column slice, converter); a slice of width one gives a scalar.
+_RECORDS = {
+    "atom": (("positions", slice(0, 3), float), ("elements", slice(3, 4), str), ("charges", slice(4, 5), float),
+             ("energies", slice(5, 6), float), ("forces", slice(6, 9), float)),
+    "lattice": (("lattice", slice(0, 3), float),),
+    "energy": (("total_energy", slice(0, 1), float),),
+    "charge": (("total_charge", slice(0, 1), float),),
+}
+
+
+def _fields(columns, rest: List[str]):
+    for name, sl, conv in columns:
+        vals = [conv(t) for t in rest[sl]]
+        yield name, vals[0] if sl.stop - sl.start == 1 else vals
 
 
 def read_runner_blocks(filename) -> Iterator[Dict[str, List]]:
-    """Yield one dict per ``begin`` ... ``end`` block (runner.py:89-123)."""
-    data: Optional[Dict[str, List]] = None
-    with open(str(filename)) as f:
-        for line in f:
-            key, tok = _tokenize(line)
-            if key == "begin":
-                data = defaultdict(list)
-            elif data is None:
-                continue
-            elif key == "atom":
-                data["positions"].append([float(t) for t in tok[:3]])
-                data["elements"].append(tok[3])
-                data["charges"].append(float(tok[4]))
-                data["energies"].append(float(tok[5]))
-                data["forces"].append([float(t) for t in tok[6:9]])
-            elif key == "lattice":
-                data["lattice"].append([float(t) for t in tok[:3]])
-            elif key == "energy":
-                data["total_energy"].append(float(tok[0]))
-            elif key == "charge":
-                data["total_charge"].append(float(tok[0]))
-            elif key == "comment":
-                data["comment"].append(" ".join(line.split()[1:]))
-            elif key == "end":
-                yield dict(data)
-                data = None
+    """Yield one dict per ``begin`` ... ``end`` block (the records of runner.py:89-123)."""
+    block: Optional[Dict[str, List]] = None
+    for raw in Path(filename).read_text().splitlines():
+        words = raw.split()
+        if not words:
+            continue
+        key, rest = words[0].lower(), words[1:]
+        if key == "begin":
+            block = defaultdict(list)
+        elif block is None:
+            continue
+        elif key == "end":
+            yield dict(block)
+            block = None
+        elif key == "comment":
+            block["comment"].append(" ".join(rest))
+        elif key in _RECORDS:
+            for name, value in _fields(_RECORDS[key], rest):
+                block[name].append(value)
 
 
 class RunnerDataSource:

@@ -341,9 +341,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--teacher-steps", type=int, default=100, help="timed steps of the mixed teacher leg")
     ap.add_argument("--fp64-steps", type=int, default=60)
-    ap.add_argument("--sweep-steps", type=int, default=200,
+    ap.add_argument("--sweep-steps", type=int, default=1000,
                     help="timed steps of the surrogate at the student's own accuracy level (0: skip)")
-    ap.add_argument("--skin", type=float, default=1.0)
+    ap.add_argument("--skin", type=float, default=0.0,
+                    help="neighbour-list skin in Bohr (0: rebuild every step, measured fastest for this teacher)")
     ap.add_argument("--tau-fs", type=float, default=0.5, help="Berendsen time constant (0: NVE)")
     ap.add_argument("--judge-every", type=int, default=20)
     ap.add_argument("--fallback-steps", type=int, default=200)

@@ -151,8 +151,11 @@ class _ForceField:
 class _DeviceMD:
     """Device-resident MD state of one System (internal atom order).
 
-    ``extra`` potentials (e.g. a surrogate and its judge) share the state and
-    the neighbour list, whose cutoff is the largest of all plus the skin.
+    ``extra`` potentials (e.g. a surrogate's judge) share the state.  The
+    neighbour list follows the driving potential's cutoff (plus the skin);
+    a potential with a longer cutoff is evaluated on an auxiliary list built
+    for the current positions when it is asked for (``list_for``), and the
+    list switches cutoff when the driver does (``use``).
 
     Periodic boxes run the step without any host read ("gated" step): the
     skin check sets a device flag, and the rebuild (pt_nbr_build_async) and
@@ -187,7 +190,8 @@ class _DeviceMD:
         for p, prec in extra:
             self.fields.setdefault((id(p), prec), _ForceField(p, prec, self.emap, n, self.dev))
         self.ff = self.fields[(id(pot), system.precision)]
-        self.rc = max(f.rc for f in self.fields.values())
+        self.rc = self.ff.rc          # the list's cutoff, without the skin: the driver's
+        self._aux = {}                # cutoff -> auxiliary list of a longer-ranged potential
         self.type_counts = s.type_counts()
         self.timers = None  # {phase: [(start_event, end_event), ...]} when profiling
         self._builds = 0
@@ -240,17 +244,10 @@ class _DeviceMD:
         self.totals.zero_()
 
     def add_field(self, pot, precision: str) -> _ForceField:
-        """Evaluate ``pot`` at ``precision`` on this state too (grows the list cutoff if needed)."""
+        """Evaluate ``pot`` at ``precision`` on this state too."""
         key = (id(pot), precision)
         if key not in self.fields:
             self.fields[key] = _ForceField(pot, precision, self.emap, self.n, self.dev)
-            rc = max(f.rc for f in self.fields.values())
-            if rc > self.rc:
-                self.rc = rc
-                pos = self._nl.pos4.clone()
-                self._nl = NeighborList(rc + self.skin, self.emap.n_types)
-                self._rebuild_sync(pos)
-                self._size_bounds()
         return self.fields[key]
 
     # the driving potential's buffers (kept for callers that read them)
@@ -270,8 +267,28 @@ class _DeviceMD:
         return self.fields[(id(pot), precision)]
 
     def use(self, field: _ForceField) -> None:
-        """Make ``field`` the driving potential of step()."""
+        """Make ``field`` the driving potential of step(); the list takes its cutoff
+        (a synchronous rebuild when the cutoff changes)."""
+        if field.rc != self.rc:
+            self.verify()
+            self.rc = field.rc
+            pos = self._nl.pos4.clone()
+            self._nl = NeighborList(self.rc + self.skin, self.emap.n_types)
+            self._rebuild_sync(pos)
+            self._size_bounds()
         self.ff = field
+
+    def list_for(self, field: _ForceField) -> NeighborList:
+        """The list to evaluate ``field`` on at the current positions: the MD's own list
+        when the field's cutoff is within the driver's, else an auxiliary list built now
+        (synchronously; its ``perm`` maps its order to the MD's internal order)."""
+        if field.rc <= self.rc:
+            return self._nl
+        aux = self._aux.get(field.rc)
+        if aux is None:
+            aux = self._aux[field.rc] = NeighborList(field.rc, self.emap.n_types)
+        aux.build(self._nl.pos4, self.types, self.box, self.type_counts, with_rev=True)
+        return aux
 
     def _mark(self, phase: str, start: bool) -> None:
         if self.timers is None:
@@ -387,9 +404,18 @@ class _DeviceMD:
 
     # ---- forces --------------------------------------------------------------
     def compute(self, out, force_mode: str, with_energy: bool = False, field: Optional[_ForceField] = None):
+        """Forces of ``field`` (default: the driver) in internal order into ``out``."""
+        import torch
+
         f = field or self.ff
         tag = "" if len(self.fields) == 1 else f"{f.label}/"
-        return f.compute(self._nl, out, force_mode, with_energy, lambda ph, st: self._mark(tag + ph, st))
+        mark = lambda ph, st: self._mark(tag + ph, st)   # noqa: E731
+        nl = self.list_for(f)
+        if nl is self._nl:
+            return f.compute(nl, out, force_mode, with_energy, mark)
+        tmp = torch.empty_like(out)
+        f.compute(nl, tmp, force_mode, with_energy, mark)
+        return scatter_rows(nl.perm, tmp, out)
 
     # ---- one step ------------------------------------------------------------
     def step(self, dt: float, integrator: str, force_mode: str, thermostat: Optional[BrendsenThermostat]):

@@ -33,9 +33,11 @@ neighbourhoods of the sample (pt_nbr_mark_rows), and the force gather then
 gives exact teacher forces on the sampled atoms.  Its cost is the share of
 atoms in those neighbourhoods instead of the whole system.
 
-Student, judge and fallback potentials share one device state and one
-neighbour list (cutoff = the largest of them + skin), so switching the
-driver costs one force evaluation and no data movement.
+Student, judge and fallback potentials share one device state.  The
+neighbour list has the driver's cutoff (plus the skin), so a student with a
+shorter cutoff than the teacher stages only its own neighbours; a judge of
+such a student builds the teacher's list for the current positions, and a
+change of driver rebuilds the list at the new driver's cutoff.
 """
 
 from __future__ import annotations
@@ -179,14 +181,20 @@ class SurrogateMD:
 
         md = self.md
         ff = self.f_judge
-        nl = md.nl
+        nl = md.list_for(ff)   # the teacher's own list when its cutoff exceeds the student's
         dev = md.dev
         lib = _lib.load()
         if self._flags is None or self._flags.numel() != md.n:
             self._flags = torch.zeros(md.n, dtype=torch.uint8, device=dev)
         self._flags.zero_()
         self.sel_dev = torch.as_tensor(np.ascontiguousarray(sel, dtype=np.int32), device=dev)
-        _lib.check(lib.pt_nbr_mark_rows(int(sel.size), self.sel_dev.data_ptr(), nl.row_off.data_ptr(),
+        sel_nl = self.sel_dev
+        if nl is not md.nl:
+            # the sampled atoms in the auxiliary list's order: inverse of its permutation
+            inv = torch.empty(md.n, dtype=torch.int32, device=dev)
+            inv[nl.perm.long()] = torch.arange(md.n, dtype=torch.int32, device=dev)
+            sel_nl = inv[self.sel_dev.long()].contiguous()
+        _lib.check(lib.pt_nbr_mark_rows(int(sel.size), sel_nl.data_ptr(), nl.row_off.data_ptr(),
                                         nl.nbr.data_ptr(), self._flags.data_ptr(), _lib.stream_ptr()),
                    "pt_nbr_mark_rows")
         cent = np.flatnonzero(self._flags.cpu().numpy()).astype(np.int32)
@@ -214,7 +222,7 @@ class SurrogateMD:
             o += k
         self.judge_centres = m
         force_gather(nl, ff.edge, ff.g_prec, out=self.f_judge_out)
-        return self.f_judge_out[self.sel_dev]
+        return self.f_judge_out[sel_nl.long()]
 
     # ---- driving -----------------------------------------------------------------
     def _to_student(self) -> None:

@@ -394,16 +394,18 @@ def test_surrogate_mixed_fallback_is_judged_and_drives():
     assert float((sur.positions - plain.positions).abs().max()) < 1e-3
 
 
-def test_surrogate_failed_mixed_fallback_escalates_to_judge_model():
+@pytest.mark.parametrize("student_rc", [12.0, 8.0])
+def test_surrogate_failed_mixed_fallback_escalates_to_judge_model(student_rc):
     """threshold 0: the mixed fallback fails its first judge too, the run rewinds and the FP64
-    model drives; the kept trajectory is the FP64 teacher's."""
+    model drives; the kept trajectory is the FP64 teacher's.  With an 8 Bohr student the list
+    switches between the student's and the teacher's cutoff at every change of driver."""
     from pantea_amd.bench_models import make_nnp, student_acsf
     from pantea_amd.simulation import MDSimulator, System
     from pantea_amd.surrogate import SurrogateMD
     from pantea_amd.units import units
 
     s, teacher = _cfg0()
-    student = make_nnp(student_acsf, (16, 16), seed=99, precision="mixed")
+    student = make_nnp(lambda e: student_acsf(e, r_cutoff=student_rc), (16, 16), seed=99, precision="mixed")
     student.fit_scaler([s])
     sur = SurrogateMD(teacher, student, s, seed=8, judge_every=4, threshold=0.0, fallback_steps=10,
                       fallback_precision="mixed", skin=1.0)
@@ -418,9 +420,12 @@ def test_surrogate_failed_mixed_fallback_escalates_to_judge_model():
     assert float((sur.positions - plain.positions).abs().max()) < 1e-9
 
 
-def test_sampled_judge_forces_equal_full_evaluation():
+@pytest.mark.parametrize("student_rc", [12.0, 8.0])
+def test_sampled_judge_forces_equal_full_evaluation(student_rc):
     """The sampled judge evaluates the teacher on the sampled atoms' neighbourhoods only; its
-    forces on those atoms equal the full evaluation's (same kernels, same FP64 sums)."""
+    forces on those atoms equal the full evaluation's (same kernels, same FP64 sums).  With an
+    8 Bohr student the MD list has the student's cutoff and both evaluations run on the
+    teacher's own list of the current positions: they equal the potential's own evaluation."""
     import torch
     from pantea_amd import water
     from pantea_amd.atoms import Structure
@@ -429,10 +434,11 @@ def test_sampled_judge_forces_equal_full_evaluation():
 
     s = Structure.from_dict(water.water_box(4096, 93.85, seed=7))
     teacher = water_nnp(s, hidden=(25, 25), seed=7, precision="float64")
-    student = make_nnp(student_acsf, (16, 16), seed=8, precision="mixed")
+    student = make_nnp(lambda e: student_acsf(e, r_cutoff=student_rc), (16, 16), seed=8, precision="mixed")
     student.fit_scaler([s])
     sur = SurrogateMD(teacher, student, s, seed=1, judge_every=10, judge_atoms=32, skin=1.0)
     md = sur.md
+    assert md.nl.r_cutoff == student_rc + 1.0
     sel = np.sort(np.random.default_rng(4).choice(md.n, 32, replace=False)).astype(np.int32)
     f_sub = sur.judge_forces(sel).cpu().numpy()
     centres = sur.judge_centres
@@ -442,6 +448,12 @@ def test_sampled_judge_forces_equal_full_evaluation():
     print(f"sampled judge: {centres} of {md.n} centres evaluated")
     assert centres < 0.6 * md.n
     np.testing.assert_allclose(f_sub, ref, rtol=0, atol=1e-12 * np.max(np.abs(ref)))
+    # the teacher's own evaluation of the same positions (input order)
+    s_now = Structure.from_dict({"positions": md.positions().cpu().numpy(), "elements": s.get_elements(),
+                                 "lattice": np.diag(s.box_lengths)})
+    _, f_pot = teacher.energy_and_forces(s_now, mode="full", precision="float64")
+    f_md = md.to_input(full).cpu().numpy()
+    np.testing.assert_allclose(f_md, f_pot.cpu().numpy(), rtol=0, atol=1e-10 * np.max(np.abs(f_md)))
     st = sur.run(30)   # an untrained student: judged once at step 10, rejected, teacher to the end
     assert st.judges == st.failures == 1 and st.fallback_steps == 30
     assert all(math.isfinite(r) for r in st.judge_rmse)

@@ -226,3 +226,16 @@ def test_units_and_masses(goldens):
     assert units.FROM_FEMTO_SECOND * 0.5 == pytest.approx(20.670687, rel=1e-6)
     em = ElementMap.from_list(["O", "H", "H"])
     assert em.element_to_atom_type == {"H": 1, "O": 2} and em.unique_elements == ("H", "O")
+
+
+def test_trainer_settings_and_updater_choice():
+    """trainer.py:29-44 (TrainingParams) and :150-171 (updater by updater_type)."""
+    from pantea_amd.kalman import NeuralNetworkPotentialTrainer, TrainingParams
+    from pantea_amd.potential import NNPSettings, NeuralNetworkPotential
+
+    s = NNPSettings.from_file(GOLDEN / "h2o.json")
+    tp = TrainingParams.from_runner(s)
+    assert (tp.force_weight, tp.energy_fraction, tp.force_fraction, tp.epochs) == (10.0, 1.0, 0.1, 20)
+    pot = NeuralNetworkPotential.from_settings(s, directory=GOLDEN)
+    with pytest.raises(NotImplementedError, match="gradient_descent"):
+        NeuralNetworkPotentialTrainer.from_settings(pot, s)   # h2o.json asks for gradient descent

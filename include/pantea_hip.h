@@ -445,6 +445,18 @@ int pt_copy_gated(long long n_words, int word_bytes, const void *src, void *dst,
  * Moves the MD state into the order of a gated rebuild. */
 int pt_permute_rows_gated(int n, int width, int word_bytes, const int *perm, void *buf, void *tmp,
                           const int *gate, pt_stream_t stream);
+/* The same for up to PT_PERMUTE_MAX arrays in two launches (gather of every
+ * array into tmp, then copy back): tmp holds n * sum(width * word_bytes)
+ * bytes (pt_permute_state_tmp_bytes). */
+#define PT_PERMUTE_MAX 8
+typedef struct {
+    void *buf;       /* (device) n rows of width words */
+    int width;
+    int word_bytes;  /* 4 or 8 */
+} pt_permute_array;
+size_t pt_permute_state_tmp_bytes(int n, int n_arrays, const pt_permute_array *arrays);
+int pt_permute_state_gated(int n, const int *perm, int n_arrays, const pt_permute_array *arrays, void *tmp,
+                           size_t tmp_bytes, const int *gate, pt_stream_t stream);
 
 /* Row gathers/scatters between input order and internal order. */
 int pt_gather_rows_f64(int n, int width, const int *idx, const double *src, double *dst,

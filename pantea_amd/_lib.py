@@ -87,6 +87,10 @@ class MlpDesc(ctypes.Structure):
     ]
 
 
+class PermuteArray(ctypes.Structure):
+    _fields_ = [("buf", _vp), ("width", _c_int), ("word_bytes", _c_int)]
+
+
 class HipLibraryError(RuntimeError):
     pass
 
@@ -144,6 +148,8 @@ _SIGS = {
     "pt_kalman_workspace_bytes": (_c_size, [_c_int, _c_int]),
     "pt_kalman_update": (_c_int, [_c_int, _c_int, _vp, _vp, _c_ll, _vp, _vp, _c_dbl, _c_dbl, _c_dbl, _vp, _c_size,
                                   _vp, _vp]),
+    "pt_permute_state_tmp_bytes": (_c_size, [_c_int, _c_int, _vp]),
+    "pt_permute_state_gated": (_c_int, [_c_int, _vp, _c_int, _vp, _vp, _c_size, _vp, _vp]),
 }
 
 #: Every symbol the header declares; the CPU test suite checks each is exported.

@@ -295,10 +295,93 @@ __global__ void k_gather_gated(const int* __restrict__ gate, int n, int width, c
     dst[t] = src[(long long)idx[i] * width + k];
 }
 
+struct PermuteArgs {
+    int n_arrays;
+    char* buf[PT_PERMUTE_MAX];
+    long long tmp_off[PT_PERMUTE_MAX];   // bytes into tmp
+    int words[PT_PERMUTE_MAX];           // words per row
+    int wbytes[PT_PERMUTE_MAX];          // 4 or 8
+};
+
+// thread per atom: row i of every array <- row perm[i], into tmp (rows written contiguously)
+__global__ void k_permute_gather(const int* __restrict__ gate, int n, const int* __restrict__ perm, PermuteArgs a,
+                                 char* __restrict__ tmp) {
+    if (gate_closed(gate)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long j = perm[i];
+    for (int k = 0; k < a.n_arrays; ++k) {
+        const int w = a.words[k];
+        if (a.wbytes[k] == 8) {
+            const double* src = reinterpret_cast<const double*>(a.buf[k]) + j * w;
+            double* dst = reinterpret_cast<double*>(tmp + a.tmp_off[k]) + (long long)i * w;
+            for (int q = 0; q < w; ++q) dst[q] = src[q];
+        } else {
+            const int* src = reinterpret_cast<const int*>(a.buf[k]) + j * w;
+            int* dst = reinterpret_cast<int*>(tmp + a.tmp_off[k]) + (long long)i * w;
+            for (int q = 0; q < w; ++q) dst[q] = src[q];
+        }
+    }
+}
+
+// every array back from tmp, one 4-byte word per thread over the concatenation
+__global__ void k_permute_copy(const int* __restrict__ gate, int n, PermuteArgs a, const char* __restrict__ tmp,
+                               long long total_words) {
+    if (gate_closed(gate)) return;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total_words;
+         t += (long long)gridDim.x * blockDim.x) {
+        // array k holds n * words[k] * wbytes[k] / 4 four-byte words starting at tmp_off[k] / 4
+        int k = 0;
+        while (k + 1 < a.n_arrays && t >= a.tmp_off[k + 1] / 4) ++k;
+        const long long u = t - a.tmp_off[k] / 4;
+        reinterpret_cast<int*>(a.buf[k])[u] = reinterpret_cast<const int*>(tmp)[t];
+    }
+}
+
 }  // namespace
 }  // namespace pt
 
 using namespace pt;
+
+extern "C" size_t pt_permute_state_tmp_bytes(int n, int n_arrays, const pt_permute_array* arrays) {
+    if (n < 0 || n_arrays < 0 || n_arrays > PT_PERMUTE_MAX || (n_arrays > 0 && !arrays)) return 0;
+    size_t b = 0;
+    for (int k = 0; k < n_arrays; ++k) b += (size_t)n * arrays[k].width * arrays[k].word_bytes;
+    return b;
+}
+
+extern "C" int pt_permute_state_gated(int n, const int* perm, int n_arrays, const pt_permute_array* arrays,
+                                      void* tmp, size_t tmp_bytes, const int* gate, pt_stream_t stream) {
+    PT_REQUIRE(n >= 0 && n_arrays >= 0 && n_arrays <= PT_PERMUTE_MAX, "pt_permute_state_gated: bad sizes");
+    if (n == 0 || n_arrays == 0) return PT_OK;
+    PT_REQUIRE(perm && arrays && tmp, "pt_permute_state_gated: null argument");
+    PermuteArgs a{};
+    a.n_arrays = n_arrays;
+    long long off = 0;
+    for (int k = 0; k < n_arrays; ++k) {
+        const pt_permute_array& r = arrays[k];
+        PT_REQUIRE(r.buf && r.width > 0 && (r.word_bytes == 4 || r.word_bytes == 8),
+                   "pt_permute_state_gated: bad array %d", k);
+        PT_REQUIRE((char*)r.buf + (size_t)n * r.width * r.word_bytes <= (char*)tmp ||
+                       (char*)tmp + tmp_bytes <= (char*)r.buf,
+                   "pt_permute_state_gated: array %d overlaps tmp", k);
+        PT_REQUIRE(off % r.word_bytes == 0, "pt_permute_state_gated: array %d would be misaligned in tmp "
+                   "(put the 8-byte arrays first)", k);
+        a.buf[k] = (char*)r.buf;
+        a.tmp_off[k] = off;
+        a.words[k] = r.width;
+        a.wbytes[k] = r.word_bytes;
+        off += (long long)n * r.width * r.word_bytes;
+    }
+    PT_REQUIRE((size_t)off <= tmp_bytes, "pt_permute_state_gated: tmp too small");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_permute_gather, dim3(nblocks(n, 256)), dim3(256), 0, st, gate, n, perm, a, (char*)tmp);
+    if (int rc = launch_check("pt_permute_state_gated")) return rc;
+    const long long words = off / 4;
+    const int blocks = (int)std::min<long long>(nblocks(words, 256), 8192);
+    hipLaunchKernelGGL(k_permute_copy, dim3(blocks), dim3(256), 0, st, gate, n, a, (const char*)tmp, words);
+    return launch_check("pt_permute_state_gated");
+}
 
 extern "C" const char* pt_last_error(void) { return pt::g_err; }
 extern "C" int pt_version(void) { return 1; }

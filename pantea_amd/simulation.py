@@ -232,6 +232,8 @@ class _DeviceMD:
         self.tmp3 = torch.empty((self.n, 3), dtype=torch.float64, device=self.dev)
         self.tmp1 = torch.empty(self.n, dtype=torch.float64, device=self.dev)
         self.tmpi = torch.empty(self.n, dtype=torch.int32, device=self.dev)
+        # the gated rebuild's permutation of v, F, m, 1/m, input index, type: 72 bytes per atom
+        self.tmp_state = torch.empty(self.n * 72, dtype=torch.uint8, device=self.dev)
         f = 2.0 if grow else 1.0
         self.row_bound = int(f * nl.max_row) + self.ROW_MARGIN
         cap = int(f * self.PAIR_MARGIN * nl.n_pairs) + 1024
@@ -348,12 +350,11 @@ class _DeviceMD:
                                           nl.row_off.data_ptr(), nl.nbr.data_ptr(), nl.rev.data_ptr(), nl.cap, g,
                                           self.totals.data_ptr(), nl.ws.data_ptr(), nl.ws.numel(), st),
                    "pt_nbr_build_async")
-        perm = nl.perm.data_ptr()
-        for buf, width, wb, tmp in ((self.vel, 3, 8, self.tmp3), (self.force, 3, 8, self.tmp3),
-                                    (self.mass, 1, 8, self.tmp1), (self.inv_mass, 1, 8, self.tmp1),
-                                    (self.orig, 1, 4, self.tmpi), (self.types, 1, 4, self.tmpi)):
-            _lib.check(lib.pt_permute_rows_gated(self.n, width, wb, perm, buf.data_ptr(), tmp.data_ptr(), g, st),
-                       "pt_permute_rows_gated")
+        arrays = (_lib.PermuteArray * 6)(*[_lib.PermuteArray(b.data_ptr(), w, wb) for b, w, wb in (
+            (self.vel, 3, 8), (self.force, 3, 8), (self.mass, 1, 8), (self.inv_mass, 1, 8), (self.orig, 1, 4),
+            (self.types, 1, 4))])
+        _lib.check(lib.pt_permute_state_gated(self.n, nl.perm.data_ptr(), 6, arrays, self.tmp_state.data_ptr(),
+                                              self.tmp_state.numel(), g, st), "pt_permute_state_gated")
         _lib.check(lib.pt_copy_gated(4 * self.n, 8, nl.pos4.data_ptr(), self.ref_pos.data_ptr(), g, st),
                    "pt_copy_gated")
         # the descriptor kernels stage rows up to the bound; buffers are sized for the capacity
@@ -445,7 +446,7 @@ class _DeviceMD:
         nl = self._nl
         bufs = (nl.pos4, nl.perm, nl.row_off, nl.nbr, nl.rev, nl.ws, self.vel, self.force, self.force_new,
                 self.mass, self.inv_mass, self.orig, self.types, self.ref_pos, self.pos_in, self.tmp3, self.tmp1,
-                self.tmpi, self.flag, self.totals, self.disp, self.reducer.ws, self.reducer.out, ff.G, ff.dEdG,
+                self.tmpi, self.tmp_state, self.flag, self.totals, self.disp, self.reducer.ws, self.reducer.out, ff.G, ff.dEdG,
                 ff.e_atom, ff.f_self, ff.edge)
         return (args[:3], id(args[3]), id(ff), id(ff.mlps), nl.cap, self.row_bound,
                 tuple(0 if b is None else b.data_ptr() for b in bufs))

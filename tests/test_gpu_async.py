@@ -145,6 +145,36 @@ def test_gated_permutation(gate_value):
         assert torch.equal(buf, want)
 
 
+@pytest.mark.parametrize("gate_value", [0, 1])
+def test_fused_state_permutation(gate_value):
+    """pt_permute_state_gated moves several arrays (FP64 and int32, widths 1 and 3) by one
+    permutation in two launches; closed gate: nothing moves."""
+    from pantea_amd import _lib
+
+    lib = _lib.load()
+    n = 1237
+    perm = torch.as_tensor(np.random.default_rng(5).permutation(n).astype(np.int32), device="cuda")
+    gate = torch.tensor([gate_value], dtype=torch.int32, device="cuda")
+    bufs = [torch.randn((n, 3), dtype=torch.float64, device="cuda"), torch.randn(n, dtype=torch.float64, device="cuda"),
+            torch.arange(n, dtype=torch.int32, device="cuda") * 7, torch.randint(0, 9, (n, 3), dtype=torch.int32,
+                                                                                 device="cuda")]
+    before = [b.clone() for b in bufs]
+    arrays = (_lib.PermuteArray * 4)(*[_lib.PermuteArray(b.data_ptr(), 1 if b.dim() == 1 else b.shape[1],
+                                                         b.element_size()) for b in bufs])
+    nbytes = lib.pt_permute_state_tmp_bytes(n, 4, arrays)
+    assert nbytes == n * (24 + 8 + 4 + 12)
+    tmp = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    _lib.check(lib.pt_permute_state_gated(n, perm.data_ptr(), 4, arrays, tmp.data_ptr(), tmp.numel(), gate.data_ptr(),
+                                          _lib.stream_ptr()), "pt_permute_state_gated")
+    torch.cuda.synchronize()
+    for b, b0 in zip(bufs, before):
+        assert torch.equal(b, b0[perm.long()] if gate_value else b0)
+    # an int32 array ahead of an FP64 one with an odd count would misalign the FP64 rows: refused
+    odd = (_lib.PermuteArray * 2)(_lib.PermuteArray(bufs[2].data_ptr(), 1, 4), _lib.PermuteArray(bufs[1].data_ptr(), 1, 8))
+    assert lib.pt_permute_state_gated(n, perm.data_ptr(), 2, odd, tmp.data_ptr(), tmp.numel(), gate.data_ptr(),
+                                      _lib.stream_ptr()) != 0
+
+
 def _trajectory(s, pot, precision, sync_free, steps=60, row_bound=None, thermostat=True, skin=1.0, graph=False):
     from pantea_amd.simulation import BrendsenThermostat, MDSimulator, System
     from pantea_amd.units import units

@@ -36,15 +36,16 @@ def test_struct_layouts_match_header():
     src = r'''
     #include <stdio.h>
     #include "pantea_hip.h"
-    int main(void){printf("%zu %zu %zu %zu\n", sizeof(pt_radial_group), sizeof(pt_angular_group),
-    sizeof(pt_sf_table), sizeof(pt_mlp_desc)); return 0;}
+    int main(void){printf("%zu %zu %zu %zu %zu\n", sizeof(pt_radial_group), sizeof(pt_angular_group),
+    sizeof(pt_sf_table), sizeof(pt_mlp_desc), sizeof(pt_permute_array)); return 0;}
     '''
     tmp = Path("/tmp/pt_sizes.c")
     tmp.write_text(src)
     subprocess.run(["gcc", "-I", str(ROOT / "include"), str(tmp), "-o", "/tmp/pt_sizes"], check=True)
     out = subprocess.run(["/tmp/pt_sizes"], capture_output=True, text=True, check=True).stdout.split()
     assert [int(x) for x in out] == [ctypes.sizeof(_lib.RadialGroup), ctypes.sizeof(_lib.AngularGroup),
-                                     ctypes.sizeof(_lib.SFTable), ctypes.sizeof(_lib.MlpDesc)]
+                                     ctypes.sizeof(_lib.SFTable), ctypes.sizeof(_lib.MlpDesc),
+                                     ctypes.sizeof(_lib.PermuteArray)]
 
 
 def test_product_fails_loudly_without_gpu():

@@ -403,6 +403,10 @@ def main() -> int:
         stud_acc = {"energy_error_mev_per_atom": abs(float(es) - float(e64)) / n * mev,
                     "force_rmse_mev_per_bohr": float(torch.sqrt(torch.mean((fs - f64) ** 2))) * mev,
                     "r_cutoff_bohr": student.r_cutoff}
+        metrics = student_dir / "metrics.json"
+        if metrics.exists():
+            val = json.loads(metrics.read_text()).get("validation_metrics", {})
+            stud_acc["validation_force_rmse_mev_per_bohr"] = val.get("force_rmse_mev_per_bohr")
         _log(f"[bench] student vs FP64 teacher on the start structure: {stud_acc}")
 
     # teacher legs
@@ -438,12 +442,15 @@ def main() -> int:
         phase_steps = min(args.steps, 200)
         _, timers_s, _ = run_surrogate(teacher, student, s, phase_steps, args.warmup, args, instrument=True)
         phases = phase_table(timers_s)
-        # the same surrogate with the judge's threshold at the student's own accuracy level
-        # (twice its force RMSE on the start structure): what the learned part buys when it drives
+        # the same surrogate with the judge's threshold at the student's own accuracy level: twice
+        # its force RMSE on held-out teacher MD snapshots (the training's validation set; else on
+        # the start structure): what the learned part buys when it drives
         if args.sweep_steps > 0 and stud_acc is not None:
-            thr = 2.0 * stud_acc["force_rmse_mev_per_bohr"]
+            basis = "validation" if stud_acc.get("validation_force_rmse_mev_per_bohr") else "start structure"
+            thr = 2.0 * (stud_acc.get("validation_force_rmse_mev_per_bohr") or stud_acc["force_rmse_mev_per_bohr"])
             dt_w, _, sw = run_surrogate(teacher, student, s, args.sweep_steps, args.warmup, args, threshold_mev=thr)
-            sweep = {"threshold_mev_per_bohr": thr, "steps": args.sweep_steps,
+            sweep = {"threshold_mev_per_bohr": thr, "threshold_basis": f"2 x the student's force RMSE ({basis})",
+                     "steps": args.sweep_steps,
                      "atom_steps_per_s": n * args.sweep_steps / dt_w, "ms_per_step": 1e3 * dt_w / args.sweep_steps,
                      "over_fp64": (n * args.sweep_steps / dt_w / fp64_value) if fp64_value else None,
                      "surrogate": sw}

@@ -325,7 +325,7 @@ __device__ __forceinline__ Screen make_screen(double thr, double bmax, const Box
     return sc;
 }
 
-constexpr int kCand = 1536;   // candidates of one cell staged in LDS (a multiple of 64)
+constexpr int kCand = 1536;   // candidates of one cell staged in LDS, large configuration (a multiple of 64)
 
 // One workgroup per cell: the candidate ranges of the cell's stencil (the
 // same for all its centres) are built once, the candidates (FP32 position and
@@ -334,8 +334,11 @@ constexpr int kCand = 1536;   // candidates of one cell staged in LDS (a multipl
 // kCand candidates reads them from global memory instead (index by binary
 // search over the ranges).  Rows come out in candidate order: type-major,
 // stencil cells ascending, i.e. ascending internal index.
-template <int PASS>
-__global__ __launch_bounds__(kCellThreads) void k_nbr_cell(const int* __restrict__ gate,
+// CAND candidates are staged in LDS per cell (a multiple of 64) and THREADS
+// threads (THREADS / 64 wavefronts) share a cell: small cells (short cutoffs)
+// take a small configuration so that more cells are resident per CU.
+template <int PASS, int CAND, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_nbr_cell(const int* __restrict__ gate,
                                                   const long long* __restrict__ total_and_max, long long cap,
                                                   const double* __restrict__ pos4,
                                                   const float4* __restrict__ pos4f,
@@ -353,7 +356,7 @@ __global__ __launch_bounds__(kCellThreads) void k_nbr_cell(const int* __restrict
     constexpr bool FILL = PASS == PASS_FILL;
     if (gated_off(gate)) return;
     if (FILL && total_and_max && (total_and_max[1] <= kStageW || total_and_max[0] > cap)) return;
-    __shared__ int4 s_cand[kCand];            // (x, y, z) as FP32 bits, internal index
+    __shared__ int4 s_cand[CAND];             // (x, y, z) as FP32 bits, internal index
     __shared__ int s_cells[32];
     __shared__ int s_runf[32], s_runl[32];    // first / last cell of each run
     __shared__ int s_rstart[kMaxRanges];
@@ -442,9 +445,9 @@ __global__ __launch_bounds__(kCellThreads) void k_nbr_cell(const int* __restrict
     __syncthreads();
     const int nr = s_nr;
     const int total = s_rpre[nr];
-    const bool staged = total <= kCand;
+    const bool staged = total <= CAND;
     if (staged) {
-        for (int t = threadIdx.x; t < total; t += kCellThreads) {
+        for (int t = threadIdx.x; t < total; t += THREADS) {
             int lo = 0, hi = nr;  // largest r with s_rpre[r] <= t
             while (hi - lo > 1) {
                 const int mid = (lo + hi) >> 1;
@@ -464,7 +467,7 @@ __global__ __launch_bounds__(kCellThreads) void k_nbr_cell(const int* __restrict
     const float3 bf = make_float3((float)box.L[0], (float)box.L[1], (float)box.L[2]);
     const float3 ib = make_float3(1.0f / bf.x, 1.0f / bf.y, 1.0f / bf.z);
     const float3 lim = make_float3(1.45f * bf.x, 1.45f * bf.y, 1.45f * bf.z);
-    for (int k = w; k < ncent; k += kCellThreads / kWave) {
+    for (int k = w; k < ncent; k += THREADS / kWave) {
         int t0 = 0;
         while (t0 + 1 < n_types && s_cpre[t0 + 1] <= k) ++t0;
         const int i = uniform(s_cbeg[t0] + (k - s_cpre[t0]));
@@ -477,7 +480,7 @@ __global__ __launch_bounds__(kCellThreads) void k_nbr_cell(const int* __restrict
             int j = i;
             float4 pj = pif;
             if (staged) {
-                // t < kCand: total <= kCand and kCand is a multiple of the wave size
+                // t < CAND: total <= CAND and CAND is a multiple of the wave size
                 const int4 c = s_cand[t];
                 pj = make_float4(__builtin_bit_cast(float, c.x), __builtin_bit_cast(float, c.y),
                                  __builtin_bit_cast(float, c.z), 0.0f);
@@ -958,10 +961,26 @@ int build_impl(int n, const double* pos, int pos_stride, const int* types, int n
 
     // count + stage, scan, then compact (or a second pass for rows > kStageW)
     const int nwb = (n + 3) / 4;
-    hipLaunchKernelGGL(k_nbr_cell<PASS_STAGE>, dim3(g.ncell), dim3(kCellThreads), 0, stream, gate,
-                       (const long long*)nullptr, o.cap, o.pos4, (const float4*)pos4f,
-                       (const unsigned long long*)bmax, cell_start, g, ba, n_types, thr, counts,
-                       (const long long*)nullptr, stage, heads);
+    // candidates per cell on average: atoms per cell times the stencil's cells; a cell
+    // above the chosen configuration's LDS stage reads its candidates from HBM instead
+    const double avg_cand = (double)n / g.ncell * g.nst[0] * g.nst[1] * g.nst[2];
+    const int ccfg = avg_cand * 1.6 <= 512 ? 0 : (avg_cand * 1.6 <= 1024 ? 1 : 2);
+    auto launch_cell = [&](bool fill, const long long* tm, const long long* roff, int* out, int* hd) {
+        auto go = [&](auto kern, int threads) {
+            hipLaunchKernelGGL(kern, dim3(g.ncell), dim3(threads), 0, stream, gate, tm, o.cap, o.pos4,
+                               (const float4*)pos4f, (const unsigned long long*)bmax, cell_start, g, ba, n_types,
+                               thr, counts, roff, out, hd);
+        };
+        if (ccfg == 0) {
+            if (fill) go(k_nbr_cell<PASS_FILL, 512, 128>, 128); else go(k_nbr_cell<PASS_STAGE, 512, 128>, 128);
+        } else if (ccfg == 1) {
+            if (fill) go(k_nbr_cell<PASS_FILL, 1024, 256>, 256); else go(k_nbr_cell<PASS_STAGE, 1024, 256>, 256);
+        } else {
+            if (fill) go(k_nbr_cell<PASS_FILL, kCand, kCellThreads>, kCellThreads);
+            else go(k_nbr_cell<PASS_STAGE, kCand, kCellThreads>, kCellThreads);
+        }
+    };
+    launch_cell(false, nullptr, nullptr, stage, heads);
     if (int rcode = launch_check("k_nbr_cell<stage>")) return rcode;
     const int nsb = (n + kScanBlock - 1) / kScanBlock;
     long long* tot_max = (long long*)misc;
@@ -980,10 +999,7 @@ int build_impl(int n, const double* pos, int pos_stride, const int* types, int n
     if (!sync) {
         // rows longer than the stage: the second pass decides on the device whether it runs
         if (o.nbr) {
-            hipLaunchKernelGGL(k_nbr_cell<PASS_FILL>, dim3(g.ncell), dim3(kCellThreads), 0, stream, gate,
-                               (const long long*)tot_max, o.cap, o.pos4, (const float4*)pos4f,
-                               (const unsigned long long*)bmax, cell_start, g, ba, n_types, thr, counts,
-                               (const long long*)o.row_off, o.nbr, (int*)nullptr);
+            launch_cell(true, (const long long*)tot_max, (const long long*)o.row_off, o.nbr, nullptr);
             if (o.rev)
                 hipLaunchKernelGGL(k_rev, dim3(nwb), dim3(256), 0, stream, gate, (const long long*)tot_max, o.cap,
                                    n, (const long long*)o.row_off, (const int*)o.nbr, o.rev);
@@ -1004,10 +1020,7 @@ int build_impl(int n, const double* pos, int pos_stride, const int* types, int n
     }
     PT_REQUIRE(o.nbr != nullptr || tm[0] == 0, "pt_nbr_build: nbr is null");
     if (tm[1] <= kStageW) return PT_OK;
-    hipLaunchKernelGGL(k_nbr_cell<PASS_FILL>, dim3(g.ncell), dim3(kCellThreads), 0, stream, gate,
-                       (const long long*)nullptr, o.cap, o.pos4, (const float4*)pos4f,
-                       (const unsigned long long*)bmax, cell_start, g, ba, n_types, thr, counts,
-                       (const long long*)o.row_off, o.nbr, (int*)nullptr);
+    launch_cell(true, nullptr, (const long long*)o.row_off, o.nbr, nullptr);
     if (int rcode = launch_check("k_nbr_cell<fill>")) return rcode;
     if (o.rev) {
         hipLaunchKernelGGL(k_rev, dim3(nwb), dim3(256), 0, stream, gate, (const long long*)nullptr, o.cap, n,

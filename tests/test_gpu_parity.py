@@ -79,9 +79,9 @@ def _rc_boundary_pairs(rc):
     return {"positions": pos, "elements": ["H"] * len(pos), "lattice": np.diag([60.0] * 3)}
 
 
-@pytest.mark.parametrize("case", ["cfg0_rc12", "box7_rc6", "box10_rc3", "cfg1_rc12", "cluster_nopbc",
+@pytest.mark.parametrize("case", ["cfg0_rc12", "box7_rc6", "box10_rc3", "cfg1_rc12", "cfg1_rc8", "cluster_nopbc",
                                   "rc_boundary_ulps", "long_rows", "cluster_far_from_origin",
-                                  "clusters_many_candidates", "dense_many_candidates"])
+                                  "clusters_many_candidates", "dense_many_candidates", "sparse_box_dense_cluster"])
 def test_neighbor_list_bit_exact(dev, case):
     from oracle import pyoracle
     from pantea_amd import water
@@ -92,6 +92,15 @@ def test_neighbor_list_bit_exact(dev, case):
         d, rc = water.config("cfg0"), 12.0
     elif case == "cfg1_rc12":
         d, rc = water.config("cfg1"), 12.0
+    elif case == "cfg1_rc8":
+        # small cells (~9 atoms, ~250 candidates): the small workgroup configuration
+        d, rc = water.config("cfg1"), 8.0
+    elif case == "sparse_box_dense_cluster":
+        # a sparse box takes the small configuration; the cluster's cells have more candidates
+        # than it stages in LDS and read them from HBM
+        pos = np.concatenate([rng.random((2000, 3)) * 60.0, 30.0 + rng.uniform(-1.5, 1.5, size=(700, 3))])
+        d = {"positions": pos, "elements": ["O", "H"] * 1350, "lattice": np.diag([60.0] * 3)}
+        rc = 4.0
     elif case == "box7_rc6":
         d = {"positions": rng.random((40, 3)) * 7.0, "elements": ["O", "H"] * 20, "lattice": np.diag([7.0] * 3)}
         rc = 6.0

@@ -994,16 +994,18 @@ __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
         }
 
         if (MODE != MODE_ADJ) {
-            // reduce only the group's own cells (uniform branches keep all lanes active)
+            // reduce only the group's cells that own a column (uniform branches
+            // keep all lanes active)
 #pragma unroll
             for (int e = 0; e < PT_AG_ETA; ++e)
 #pragma unroll
-                for (int q = 0; q < PT_AG_ANG; ++q)
-                    if (e < X.n_eta && q < X.n_ang) {
+                for (int q = 0; q < PT_AG_ANG; ++q) {
+                    const int col = (e < X.n_eta && q < X.n_ang) ? uniform(G->col[e * PT_AG_ANG + qmap[q]]) : -1;
+                    if (col >= 0) {
                         const R v = wave_sum(acc[e][q]);
-                        const int col = G->col[e * PT_AG_ANG + qmap[q]];
-                        if (lane == 0 && col >= 0) out[(size_t)c * a.ld + col] = v;
+                        if (lane == 0) out[(size_t)c * a.ld + col] = v;
                     }
+                }
         }
         wave_sync();
     }

@@ -184,6 +184,30 @@ class SFRow:
 TANH_PRE = ((math.e + 1 / math.e) / (math.e - 1 / math.e)) ** 3  # cutoff.py:82
 
 
+_Z14_TERMS = ((1.0, 1.0), (1.0, -1.0), (4.0, 1.0), (4.0, -1.0))
+_TANH_CUTS = (3, 2)   # PT_CUT_TANH, PT_CUT_TANHU
+
+
+def _pad_to_z14(angular: List[dict]) -> None:
+    """Complete every angular group to the four terms zeta {1, 4} x lambda +-1
+    (the added terms own no column) when each group's terms are a subset of
+    them under a tanh cutoff and the table is not of the cheaper class with
+    at most two zeta = 1 terms per group.  The table then runs the kernels of
+    the four-term class (shared angular powers, the quartic adjoint), whose
+    per-pair cost is below that of the general integer-zeta class; the
+    descriptor values do not change."""
+    if not angular:
+        return
+    z1n2 = all(len(g["angs"]) <= 2 and all(z == 1.0 for z, _ in g["angs"]) for g in angular)
+    subset = all(g["key"][2] in _TANH_CUTS and all(a in _Z14_TERMS for a in g["angs"]) for g in angular)
+    if z1n2 or not subset:
+        return
+    for g in angular:
+        for a in _Z14_TERMS:
+            if a not in g["angs"]:
+                g["angs"].append(a)
+
+
 def compile_table(rows: Sequence[SFRow]) -> _lib.SFTable:
     """Group the functions into the radial/angular groups of pt_sf_table."""
     t = _lib.SFTable()
@@ -227,6 +251,7 @@ def compile_table(rows: Sequence[SFRow]) -> _lib.SFTable:
             raise ValueError(f"unknown symmetry function kind {r.kind}")
     if len(radial) > _lib.PT_MAX_GROUPS or len(angular) > _lib.PT_MAX_GROUPS:
         raise ValueError("too many symmetry-function groups for one descriptor")
+    _pad_to_z14(angular)
     t.n_rg = len(radial)
     for gi, g in enumerate(radial):
         G = t.rg[gi]

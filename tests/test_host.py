@@ -126,7 +126,17 @@ def test_table_kind_classification():
     z14 = ((1.0, 1.0), (1.0, -1.0), (4.0, 1.0), (4.0, -1.0))
     assert kind(angular(z14)) == _lib.PT_ACSF_TANH_Z14
     assert kind(angular(z14[::-1])) == _lib.PT_ACSF_TANH_Z14           # any order of the four terms
-    assert kind(angular(z14[:3])) == _lib.PT_ACSF_TANH_P4              # three of them
+    # three of them: the compiler completes the group with a term that owns no column
+    assert kind(angular(z14[:3])) == _lib.PT_ACSF_TANH_Z14
+    t = compile_table(angular(z14[:3]).rows(emap))
+    g = t.ag[0]
+    assert g.n_ang == 4 and t.n_sf == 7
+    assert sorted(c for c in g.col if c >= 0) == list(range(1, 7))
+    assert [g.col[e * _lib.PT_AG_ANG + 3] for e in range(2)] == [-1, -1]
+    assert (g.zeta[3], g.lam[3]) == (4.0, -1.0)
+    # not when a group has another term, nor for the cheaper zeta = 1 class
+    assert kind(angular(z14[:2])) == _lib.PT_ACSF_TANH_Z1N2
+    assert compile_table(angular(z14[:2]).rows(emap)).ag[0].n_ang == 2
     assert kind(angular(((1.0, 1.0), (2.0, -1.0), (4.0, 1.0), (4.0, -1.0)))) == _lib.PT_ACSF_TANH_P4
     z3 = ACSF("H", ((G2(cfn, 0.0, 0.1), NeighborElements("H")),),
               ((G3(cfn, eta=0.1, zeta=3.0, lambda0=1.0), NeighborElements("H", "H")),))

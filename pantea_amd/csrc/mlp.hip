@@ -316,13 +316,115 @@ __device__ __forceinline__ void gemm_tile(const typename MT<MODE>::T* A, int str
     (void)bt;
 }
 
+// C(16 x 16 nt) += A(16 x K, LDS row-major) * B over the column tiles
+// n0 + 16 t, t < nt <= 2, of one layer.  The A fragment of a k step is read
+// and split once for all tiles, and the weight fragments of all tiles are
+// loaded before their products: the wave waits on weight loads once per k
+// step rather than once per tile and step.  Per tile the products run in the
+// same k order as gemm_tile.  Two tiles per pass: four held the weight
+// fragments of too many tiles at once for four resident waves per SIMD.
+template <int MODE>
+__device__ __forceinline__ void gemm_cols(const typename MT<MODE>::T* A, int stride, int K, const char* packed,
+                                          size_t off_hi, size_t off_lo, int ldb, bool bt, int n0, int nt,
+                                          typename MT<MODE>::T (&acc)[2][4]) {
+    const int l = wave_lane();
+    const int row = l & 15, grp = l >> 4;
+    if constexpr (MODE == PT_MLP_F64) {
+        pt_d4 c[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) c[t] = pt_d4{acc[t][0], acc[t][1], acc[t][2], acc[t][3]};
+        const double* W = (const double*)(packed + off_hi);
+#pragma unroll 2
+        for (int k0 = 0; k0 < K; k0 += 4) {
+            const int k = k0 + grp;
+            const double av = A[row * stride + k];
+            double b[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int n = n0 + 16 * t + row;
+                b[t] = t < nt ? (bt ? W[(size_t)n * ldb + k] : W[(size_t)k * ldb + n]) : 0.0;
+            }
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+                if (t < nt) c[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[t], c[t], 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            acc[t][0] = c[t][0]; acc[t][1] = c[t][1]; acc[t][2] = c[t][2]; acc[t][3] = c[t][3];
+        }
+    } else {
+        pt_f4 c[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) c[t] = pt_f4{acc[t][0], acc[t][1], acc[t][2], acc[t][3]};
+        const unsigned short* Bh = (const unsigned short*)(packed + off_hi);
+        const unsigned short* Bl = (const unsigned short*)(packed + off_lo);
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            const int kb = k0 + 8 * grp;
+            // B fragments: 8 consecutive k of column n0 + 16 t + row (k-contiguous layout)
+            uint4 vh[2], vl[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                vh[t] = make_uint4(0u, 0u, 0u, 0u);
+                vl[t] = vh[t];
+                if (t < nt) {
+                    const size_t bo = (size_t)(n0 + 16 * t + row) * ldb + kb;
+                    vh[t] = *reinterpret_cast<const uint4*>(Bh + bo);
+                    if (MODE == PT_MLP_BF16X3) vl[t] = *reinterpret_cast<const uint4*>(Bl + bo);
+                }
+            }
+            pt_bf8 ah, al;
+            const float* ap = A + row * stride + kb;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                unsigned short hi, lo;
+                split_bf16(ap[q], hi, lo);
+                ah[q] = __builtin_bit_cast(__bf16, hi);
+                al[q] = __builtin_bit_cast(__bf16, lo);
+            }
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                if (t < nt) {
+                    const pt_bf8 bh = __builtin_bit_cast(pt_bf8, vh[t]);
+                    const pt_bf8 bl = __builtin_bit_cast(pt_bf8, vl[t]);
+                    c[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c[t], 0, 0, 0);
+                    if (MODE == PT_MLP_BF16X3) {
+                        c[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c[t], 0, 0, 0);
+                        c[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c[t], 0, 0, 0);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            acc[t][0] = c[t][0]; acc[t][1] = c[t][1]; acc[t][2] = c[t][2]; acc[t][3] = c[t][3];
+        }
+    }
+    (void)bt;
+}
+
+// Lanes over the cells of a 16-row tile, for elementwise passes: a chunk of 64
+// columns gives lane l column k0 + l of all 16 rows; a last chunk of 32 columns
+// (widths are multiples of 32) gives lanes 32-63 the odd rows, 8 rows each.
+struct TileCols {
+    int k;       // this lane's column
+    int r0;      // first row
+    int rstep;   // row step
+    int nr;      // rows per lane (16 or 8)
+};
+
+__device__ __forceinline__ TileCols tile_cols(int k0, int width) {
+    const int l = wave_lane();
+    if (width - k0 >= kWave) return TileCols{k0 + l, 0, 1, 16};
+    return TileCols{k0 + (l & 31), l >> 5, 2, 8};
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
     typedef typename MT<MODE>::T T;
     extern __shared__ __align__(16) char smem[];
     const MlpShape& s = a.s;
     const int l = wave_lane();
-    const int wv = threadIdx.x >> 6;
+    const int wv = uniform(threadIdx.x >> 6);
     const int wpb = blockDim.x >> 6;
     const int stride = a.stride;
     T* base = reinterpret_cast<T*>(smem) + (size_t)wv * ((size_t)a.nbuf * 16 * stride + 16);
@@ -338,60 +440,91 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
     for (int h = 0; h < kMaxHidden; ++h)
         D[h] = base + (size_t)(a.inplace ? ys(h) : ny + h) * 16 * stride;
     T* gout = base + (size_t)a.nbuf * 16 * stride;  // 16 output derivatives
+    const T* wout = (MODE == PT_MLP_F64) ? (const T*)(a.packed + s.wout64) : (const T*)(a.packed + s.wout32);
+    const T bout = (MODE == PT_MLP_F64) ? (T)((const double*)(a.packed + s.bout64))[0]
+                                        : (T)((const float*)(a.packed + s.bout32))[0];
 
     const int row = l & 15, grp = l >> 4;
     const int ntiles = (a.n_atoms + 15) / 16;
     for (int tile = blockIdx.x * wpb + wv; tile < ntiles; tile += gridDim.x * wpb) {
         const int a0 = tile * 16;
-        // 1. scaled input -> slot 0 (zero padded)
-        for (int e = l; e < 16 * s.in_p; e += kWave) {
-            const int r = e / s.in_p, k = e % s.in_p;
-            const int atom = a0 + r;
-            T v = T(0);
-            if (atom < a.n_atoms && k < s.in_dim) {
-                const double g = a.g_f64 ? ((const double*)a.G)[(size_t)atom * a.ldg + k]
-                                         : (double)((const float*)a.G)[(size_t)atom * a.ldg + k];
-                v = (T)((g - a.scale[k]) * a.scale[s.in_dim + k] + a.scale_add);
+        // 1. scaled input -> slot 0 (zero padded): every row's load of a
+        //    column is issued before the first store
+        for (int k0 = 0; k0 < s.in_p; k0 += kWave) {
+            const TileCols tc = tile_cols(k0, s.in_p);
+            const int k = tc.k;
+            const bool col_ok = k < s.in_dim;
+            const double sub = col_ok ? a.scale[k] : 0.0;
+            const double mul = col_ok ? a.scale[s.in_dim + k] : 0.0;
+            for (int j0 = 0; j0 < tc.nr; j0 += 8) {
+                double g[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int atom = a0 + tc.r0 + (j0 + j) * tc.rstep;
+                    g[j] = 0.0;
+                    if (col_ok && atom < a.n_atoms)
+                        g[j] = a.g_f64 ? ((const double*)a.G)[(size_t)atom * a.ldg + k]
+                                       : (double)((const float*)a.G)[(size_t)atom * a.ldg + k];
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int r = tc.r0 + (j0 + j) * tc.rstep;
+                    base[r * stride + k] = (col_ok && a0 + r < a.n_atoms) ? (T)((g[j] - sub) * mul + a.scale_add)
+                                                                         : T(0);
+                }
             }
-            base[r * stride + k] = v;
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
 
-        // 2. hidden layers
+        // 2. hidden layers, 32 columns (two MFMA tiles) at a time
         int kin = s.in_p;
         T* xin = base;
         for (int h = 0; h < H; ++h) {
             const int wp = s.wp[h];
+            const int act = s.act[h];
             T* yout = base + (size_t)ys(h) * 16 * stride;
-            for (int n0 = 0; n0 < wp; n0 += 16) {
-                T acc[4] = {T(0), T(0), T(0), T(0)};
+            for (int n0 = 0; n0 < wp; n0 += 32) {
+                const int nt = min(2, (wp - n0) >> 4);
+                T acc[2][4] = {};
                 if (MODE == PT_MLP_F64)
-                    gemm_tile<MODE>(xin, stride, kin, a.packed, s.W64[h], 0, wp, false, n0, acc);
+                    gemm_cols<MODE>(xin, stride, kin, a.packed, s.W64[h], 0, wp, false, n0, nt, acc);
                 else
-                    gemm_tile<MODE>(xin, stride, kin, a.packed, s.WTh[h], s.WTl[h], kin, true, n0, acc);
-                const int col = n0 + row;
-                const T bias = (MODE == PT_MLP_F64) ? (T)((const double*)(a.packed + s.b64[h]))[col]
-                                                    : (T)((const float*)(a.packed + s.b32[h]))[col];
+                    gemm_cols<MODE>(xin, stride, kin, a.packed, s.WTh[h], s.WTl[h], kin, true, n0, nt, acc);
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int r = crow<MODE>(grp, i);
-                    T y, dy;
-                    activate<T>(s.act[h], acc[i] + bias, y, dy);
-                    if (col >= s.w[h]) { y = T(0); dy = T(0); }
-                    yout[r * stride + col] = y;
-                    if (!a.inplace) D[h][r * stride + col] = dy;
+                for (int t = 0; t < 2; ++t) {
+                    if (t < nt) {
+                        const int col = n0 + 16 * t + row;
+                        const T bias = (MODE == PT_MLP_F64) ? (T)((const double*)(a.packed + s.b64[h]))[col]
+                                                            : (T)((const float*)(a.packed + s.b32[h]))[col];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const int r = crow<MODE>(grp, i);
+                            T y, dy;
+                            activate<T>(act, acc[t][i] + bias, y, dy);
+                            if (col >= s.w[h]) { y = T(0); dy = T(0); }
+                            yout[r * stride + col] = y;
+                            if (!a.inplace) D[h][r * stride + col] = dy;
+                        }
+                    }
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
             if (a.inplace && h > 0) {
                 // layer h has read the output of layer h - 1: turn it into its derivative
-                const int wq = s.wp[h - 1];
-                for (int e = l; e < 16 * wq; e += kWave) {
-                    const int r = e / wq, k = e % wq;
-                    T* p = D[h - 1] + r * stride + k;
-                    *p = k < s.w[h - 1] ? dact_from_output<T>(s.act[h - 1], *p) : T(0);
+                const int wq = s.wp[h - 1], wt = s.w[h - 1], aq = s.act[h - 1];
+                for (int k0 = 0; k0 < wq; k0 += kWave) {
+                    const TileCols tc = tile_cols(k0, wq);
+                    for (int j0 = 0; j0 < tc.nr; j0 += 8) {
+                        T v[8];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) v[j] = D[h - 1][(tc.r0 + (j0 + j) * tc.rstep) * stride + tc.k];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j)
+                            D[h - 1][(tc.r0 + (j0 + j) * tc.rstep) * stride + tc.k] =
+                                tc.k < wt ? dact_from_output<T>(aq, v[j]) : T(0);
+                    }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -401,15 +534,22 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
         }
         T* xlast = xin;
 
-        // 3. output layer (width 1): 4 lanes per atom
+        // 3. output layer (width 1): 4 lanes per atom, lane part takes k = part + 4 j
         {
-            const T* wout = (MODE == PT_MLP_F64) ? (const T*)(a.packed + s.wout64)
-                                                 : (const T*)(a.packed + s.wout32);
-            const T bout = (MODE == PT_MLP_F64) ? (T)((const double*)(a.packed + s.bout64))[0]
-                                                : (T)((const float*)(a.packed + s.bout32))[0];
             const int r = l >> 2, part = l & 3;
             T sum = T(0);
-            for (int k = part; k < kin; k += 4) sum += xlast[r * stride + k] * wout[k];
+            for (int j0 = 0; j0 < kin; j0 += 32) {
+                T w[8], x[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int k = j0 + part + 4 * j;
+                    w[j] = k < kin ? wout[k] : T(0);
+                    x[j] = k < kin ? xlast[r * stride + k] : T(0);
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (j0 + part + 4 * j < kin) sum += x[j] * w[j];
+            }
             sum += __shfl_xor(sum, 1, kWave);
             sum += __shfl_xor(sum, 2, kWave);
             T y, dy;
@@ -424,55 +564,84 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
         if (!a.dEdG) continue;
 
         // 4. backward: dZ of the last hidden layer in place in D[h]
-        const T* wout = (MODE == PT_MLP_F64) ? (const T*)(a.packed + s.wout64)
-                                             : (const T*)(a.packed + s.wout32);
         if (s.n_hidden > 0) {
             const int h = s.n_hidden - 1;
-            for (int e = l; e < 16 * s.wp[h]; e += kWave) {
-                const int r = e / s.wp[h], k = e % s.wp[h];
-                T* p = D[h] + r * stride + k;
-                // in place: the slot still holds the last hidden output
-                const T dy = !a.inplace ? *p : (k < s.w[h] ? dact_from_output<T>(s.act[h], *p) : T(0));
-                *p = dy * gout[r] * wout[k];
+            const int wq = s.wp[h], wt = s.w[h], aq = s.act[h];
+            for (int k0 = 0; k0 < wq; k0 += kWave) {
+                const TileCols tc = tile_cols(k0, wq);
+                const T wk = wout[tc.k];
+                for (int j0 = 0; j0 < tc.nr; j0 += 8) {
+                    T v[8], go[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const int r = tc.r0 + (j0 + j) * tc.rstep;
+                        v[j] = D[h][r * stride + tc.k];
+                        go[j] = gout[r];
+                    }
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        // in place: the slot still holds the last hidden output
+                        const T dy = !a.inplace ? v[j] : (tc.k < wt ? dact_from_output<T>(aq, v[j]) : T(0));
+                        D[h][(tc.r0 + (j0 + j) * tc.rstep) * stride + tc.k] = dy * go[j] * wk;
+                    }
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
             for (int hh = h; hh >= 1; --hh) {
                 // dZ_{hh-1} = (dZ_hh . W_hh^T) * D[hh-1]
                 const int wprev = s.wp[hh - 1];
-                for (int n0 = 0; n0 < wprev; n0 += 16) {
-                    T acc[4] = {T(0), T(0), T(0), T(0)};
+                for (int n0 = 0; n0 < wprev; n0 += 32) {
+                    const int nt = min(2, (wprev - n0) >> 4);
+                    T acc[2][4] = {};
                     if (MODE == PT_MLP_F64)
-                        gemm_tile<MODE>(D[hh], stride, s.wp[hh], a.packed, s.W64[hh], 0, s.wp[hh], true, n0, acc);
+                        gemm_cols<MODE>(D[hh], stride, s.wp[hh], a.packed, s.W64[hh], 0, s.wp[hh], true, n0, nt, acc);
                     else
-                        gemm_tile<MODE>(D[hh], stride, s.wp[hh], a.packed, s.Wh[hh], s.Wl[hh], s.wp[hh], true, n0, acc);
+                        gemm_cols<MODE>(D[hh], stride, s.wp[hh], a.packed, s.Wh[hh], s.Wl[hh], s.wp[hh], true, n0,
+                                        nt, acc);
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) D[hh - 1][crow<MODE>(grp, i) * stride + n0 + row] *= acc[i];
+                    for (int t = 0; t < 2; ++t)
+                        if (t < nt) {
+#pragma unroll
+                            for (int i = 0; i < 4; ++i)
+                                D[hh - 1][crow<MODE>(grp, i) * stride + n0 + 16 * t + row] *= acc[t][i];
+                        }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 __builtin_amdgcn_wave_barrier();
             }
         }
         // 5. dE/dx' = dZ_0 . W_0^T (or wout for a linear model), then chain the scaler
-        for (int n0 = 0; n0 < s.in_p; n0 += 16) {
-            T acc[4] = {T(0), T(0), T(0), T(0)};
+        for (int n0 = 0; n0 < s.in_p; n0 += 32) {
+            const int nt = min(2, (s.in_p - n0) >> 4);
+            T acc[2][4] = {};
             if (s.n_hidden > 0) {
                 if (MODE == PT_MLP_F64)
-                    gemm_tile<MODE>(D[0], stride, s.wp[0], a.packed, s.W64[0], 0, s.wp[0], true, n0, acc);
+                    gemm_cols<MODE>(D[0], stride, s.wp[0], a.packed, s.W64[0], 0, s.wp[0], true, n0, nt, acc);
                 else
-                    gemm_tile<MODE>(D[0], stride, s.wp[0], a.packed, s.Wh[0], s.Wl[0], s.wp[0], true, n0, acc);
+                    gemm_cols<MODE>(D[0], stride, s.wp[0], a.packed, s.Wh[0], s.Wl[0], s.wp[0], true, n0, nt, acc);
             } else {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) acc[i] = gout[crow<MODE>(grp, i)] * wout[n0 + row];
-            }
-            const int k = n0 + row;
+                for (int t = 0; t < 2; ++t)
+                    if (t < nt) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int atom = a0 + crow<MODE>(grp, i);
-                if (atom < a.n_atoms && k < s.in_dim) {
-                    const double gk = (double)acc[i] * a.scale[s.in_dim + k];
-                    if (a.g_f64) ((double*)a.dEdG)[(size_t)atom * a.ldg + k] = gk;
-                    else ((float*)a.dEdG)[(size_t)atom * a.ldg + k] = (float)gk;
+                        for (int i = 0; i < 4; ++i) acc[t][i] = gout[crow<MODE>(grp, i)] * wout[n0 + 16 * t + row];
+                    }
+            }
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int k = n0 + 16 * t + row;
+                if (t < nt && k < s.in_dim) {
+                    const double mk = a.scale[s.in_dim + k];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int atom = a0 + crow<MODE>(grp, i);
+                        if (atom < a.n_atoms) {
+                            const double gk = (double)acc[t][i] * mk;
+                            if (a.g_f64) ((double*)a.dEdG)[(size_t)atom * a.ldg + k] = gk;
+                            else ((float*)a.dEdG)[(size_t)atom * a.ldg + k] = (float)gk;
+                        }
+                    }
                 }
             }
         }

@@ -359,6 +359,8 @@ __global__ __launch_bounds__(THREADS) void k_nbr_cell(const int* __restrict__ ga
     __shared__ int4 s_cand[CAND];             // (x, y, z) as FP32 bits, internal index
     __shared__ int s_cells[32];
     __shared__ int s_runf[32], s_runl[32];    // first / last cell of each run
+    __shared__ int s_cimg[32], s_rimg[32];    // periodic image of each stencil cell / run (code below)
+    __shared__ int s_nrun;
     __shared__ int s_rstart[kMaxRanges];
     __shared__ int s_rpre[kMaxRanges + 1];
     __shared__ int s_cpre[kMaxTypes + 1];     // centres of this cell, by type
@@ -389,11 +391,16 @@ __global__ __launch_bounds__(THREADS) void k_nbr_cell(const int* __restrict__ ga
         // stencil cells, sorted ascending
         const int ns = g.nst[0] * g.nst[1] * g.nst[2];
         int myc = -1;
+        int img = 13;   // (sx + 1) + 3 (sy + 1) + 9 (sz + 1): the cell's positions are seen shifted by s L
         if (lane < ns) {
             int a = lane % g.nst[0], b = (lane / g.nst[0]) % g.nst[1], c = lane / (g.nst[0] * g.nst[1]);
             int x = c0 + g.off[0][a], y = c1 + g.off[1][b], z = c2 + g.off[2][c];
             bool ok = true;
             if (g.periodic) {
+                const int sx = x < 0 ? -1 : (x >= g.nc[0] ? 1 : 0);
+                const int sy = y < 0 ? -1 : (y >= g.nc[1] ? 1 : 0);
+                const int sz = z < 0 ? -1 : (z >= g.nc[2] ? 1 : 0);
+                img = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
                 x = (x + g.nc[0]) % g.nc[0]; y = (y + g.nc[1]) % g.nc[1]; z = (z + g.nc[2]) % g.nc[2];
             } else {
                 ok = x >= 0 && x < g.nc[0] && y >= 0 && y < g.nc[1] && z >= 0 && z < g.nc[2];
@@ -406,17 +413,20 @@ __global__ __launch_bounds__(THREADS) void k_nbr_cell(const int* __restrict__ ga
             rank += (o >= 0 && o < myc) ? 1 : 0;
         }
         const int nvalid = __popcll(__ballot(myc >= 0));
-        if (myc >= 0) s_cells[rank] = myc;
+        if (myc >= 0) { s_cells[rank] = myc; s_cimg[rank] = img; }
         wave_sync();
-        // runs of consecutive cell ids (x-adjacent cells) are contiguous per type
+        // runs of consecutive cell ids (x-adjacent cells) are contiguous per type; a
+        // run also keeps to one periodic image (the last cell of a row and the first
+        // of the next have consecutive ids but are seen in different images)
         const bool in = lane < nvalid;
         const int cv = in ? s_cells[lane] : 0;
-        const bool first = in && (lane == 0 || s_cells[lane - 1] != cv - 1);
-        const bool last = in && (lane + 1 == nvalid || s_cells[lane + 1] != cv + 1);
+        const int ci = in ? s_cimg[lane] : 0;
+        const bool first = in && (lane == 0 || s_cells[lane - 1] != cv - 1 || s_cimg[lane - 1] != ci);
+        const bool last = in && (lane + 1 == nvalid || s_cells[lane + 1] != cv + 1 || s_cimg[lane + 1] != ci);
         const unsigned long long fm = __ballot(first);
         const int nrun = __popcll(fm);
         const int run = __popcll(fm & ((2ull << lane) - 1ull)) - 1;   // lane < 32
-        if (first) s_runf[run] = cv;
+        if (first) { s_runf[run] = cv; s_rimg[run] = ci; }
         if (last) s_runl[run] = cv;
         wave_sync();
         // ranges (type-major, runs ascending) and their inclusive prefix
@@ -440,13 +450,22 @@ __global__ __launch_bounds__(THREADS) void k_nbr_cell(const int* __restrict__ ga
             if (r < nr) { s_rstart[r] = st; s_rpre[r + 1] = carry + incl; }
             carry += __shfl(incl, kWave - 1, kWave);
         }
-        if (lane == 0) { s_rpre[0] = 0; s_nr = nr; }
+        if (lane == 0) { s_rpre[0] = 0; s_nr = nr; s_nrun = nrun; }
     }
     __syncthreads();
     const int nr = s_nr;
     const int total = s_rpre[nr];
     const bool staged = total <= CAND;
+    // With at least 4 cells per dimension the cells of one run (x-consecutive ids)
+    // share one periodic image, and a pair within the cutoff is always seen in the
+    // image of the stencil cell that holds it (cells are at least rc wide): the
+    // staged candidates are then stored at that image and the distance test needs
+    // no per-pair image.  A pair seen in another image is farther apart than its
+    // minimum image, i.e. beyond the cutoff either way.
+    const bool direct = staged && (!box.periodic || (g.nc[0] >= 4 && g.nc[1] >= 4 && g.nc[2] >= 4));
+    const float3 bf = make_float3((float)box.L[0], (float)box.L[1], (float)box.L[2]);
     if (staged) {
+        const int nrun = s_nrun;
         for (int t = threadIdx.x; t < total; t += THREADS) {
             int lo = 0, hi = nr;  // largest r with s_rpre[r] <= t
             while (hi - lo > 1) {
@@ -454,7 +473,14 @@ __global__ __launch_bounds__(THREADS) void k_nbr_cell(const int* __restrict__ ga
                 if (s_rpre[mid] <= t) lo = mid; else hi = mid;
             }
             const int j = s_rstart[lo] + (t - s_rpre[lo]);
-            const float4 p = pos4f[j];
+            float4 p = pos4f[j];
+            if (direct && box.periodic) {
+                const int im = s_rimg[lo % nrun];
+                const int sx = im % 3 - 1, sy = (im / 3) % 3 - 1, sz = im / 9 - 1;
+                p.x += (float)sx * bf.x;
+                p.y += (float)sy * bf.y;
+                p.z += (float)sz * bf.z;
+            }
             s_cand[t] = make_int4(__builtin_bit_cast(int, p.x), __builtin_bit_cast(int, p.y),
                                   __builtin_bit_cast(int, p.z), j);
         }
@@ -464,7 +490,6 @@ __global__ __launch_bounds__(THREADS) void k_nbr_cell(const int* __restrict__ ga
     const Screen sc = make_screen(thr, __builtin_bit_cast(double, *bmax), box);
     // FP32 image: d - L rint(d / L), the single-shift image whenever |d| < 1.5 L
     // (a candidate with a larger component takes the FP64 test)
-    const float3 bf = make_float3((float)box.L[0], (float)box.L[1], (float)box.L[2]);
     const float3 ib = make_float3(1.0f / bf.x, 1.0f / bf.y, 1.0f / bf.z);
     const float3 lim = make_float3(1.45f * bf.x, 1.45f * bf.y, 1.45f * bf.z);
     for (int k = w; k < ncent; k += THREADS / kWave) {
@@ -497,7 +522,7 @@ __global__ __launch_bounds__(THREADS) void k_nbr_cell(const int* __restrict__ ga
             const bool ok = (t < total) & (j != i);   // r_ii = 0 is never a neighbour
             float dx = pif.x - pj.x, dy = pif.y - pj.y, dz = pif.z - pj.z;
             bool img = true;
-            if (box.periodic) {
+            if (box.periodic && !direct) {
                 img = (fabsf(dx) < lim.x) & (fabsf(dy) < lim.y) & (fabsf(dz) < lim.z);
                 dx = fmaf(-bf.x, rintf(dx * ib.x), dx);
                 dy = fmaf(-bf.y, rintf(dy * ib.y), dy);

@@ -52,6 +52,7 @@ sys.path.insert(0, str(ROOT))
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP64_MFMA_TFLOPS = 78.6   # MI355X dense FP64 matrix
 kWaveThreads = 64   # a descriptor launch runs one wavefront per centre
 
 
@@ -325,6 +326,95 @@ def run_surrogate(teacher, student, s, steps, warmup, args, instrument=False, th
     return dt, timers, delta
 
 
+def kalman_flops(n_w: int, m: int) -> float:
+    """Algorithmic FP64 flops of one pt_kalman_update in the form it computes.  m <= n_w: X = P H
+    (2 n_w^2 m), A = H^T X (2 m^2 n_w), Cholesky of A (m^3 / 3), the solve for Y = A^-1 X^T (2 m^2 n_w),
+    P -= Y^T X^T (2 n_w^2 m).  m > n_w (n_w x n_w factors): Cholesky of P (n_w^3 / 3), G = L^T H
+    (n_w^2 m), S = G G^T (n_w^2 m), Cholesky of S (n_w^3 / 3), M = L Ls^-T (n_w^3), M M^T (n_w^3)."""
+    if m > n_w:
+        return 2.0 * n_w * n_w * m + 8.0 / 3.0 * n_w ** 3
+    return 4.0 * n_w * n_w * m + 4.0 * m * m * n_w + m ** 3 / 3.0
+
+
+def run_kalman(args):
+    """The Kalman-filter trainer's per-structure cost on the student network (34-16-16-1 per element):
+    one force update of a 512-molecule box (1,536 atoms, m = 4,608 observations) and one energy update,
+    device time per call from HIP events; plus the same force update on the host with numpy (the
+    reference's formula, kalman_filter.py:196-233) as the CPU figure."""
+    import numpy as np
+    import torch
+
+    from pantea_amd import bench_models, water
+    from pantea_amd.atoms import Structure
+    from pantea_amd.kalman import KalmanFilter
+    from pantea_amd.potential import NNPSettings
+
+    s = Structure.from_dict(water.water_box(512, 187.7 / 4, seed=5))
+    student = bench_models.water_student(None, seed=12, precision="float64")
+    student.fit_scaler([s])
+    E, F = student.energy_and_forces(s, mode="reference", precision="float64")
+    s.forces = F + 0.01 * torch.sin(s.positions)
+    s.total_energy = E + 0.5
+    settings = NNPSettings(elements=list(student.elements), global_nodes_short=[16, 16],
+                           global_activation_short=["tanh", "tanh", "identity"], symfunction_short=[])
+    kf = KalmanFilter.from_settings(student, settings)
+    kf.dev = torch.device("cuda")
+    kf._bind()
+    items = kf._prepare([s])
+    kf._buffers(items)
+    it = items[0]
+    n_w = kf.num_states
+
+    def timed(fn, reps):
+        fn()
+        _sync()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        _sync()
+        return a.elapsed_time(b) / reps
+
+    reps = max(1, args.kalman_updates)
+    jac_ms = timed(lambda: kf.calculate_loss_force(it), reps)
+    xi, H = kf.calculate_loss_force(it)
+    xi, H = xi.clone(), H.clone()
+    m = int(xi.numel())
+    upd_ms = timed(lambda: kf.update_state_vector(xi, H), reps)
+    xe, He = kf.calculate_loss_energy(it)
+    xe, He = xe.clone(), He.clone()
+    en_ms = timed(lambda: (kf.calculate_loss_energy(it), kf.update_state_vector(xe, He)), reps)
+    if int(kf.info_sum.item()) != 0:
+        raise RuntimeError("Kalman update: a Cholesky factorisation failed")
+    flops = kalman_flops(n_w, m)
+    out = {"network": "student 34-16-16-1 per element (H, O)", "atoms": s.natoms, "n_weights": n_w,
+           "observations": m, "force_jacobian_ms": jac_ms, "force_update_ms": upd_ms,
+           "energy_observation_and_update_ms": en_ms, "update_flops": flops,
+           "update_tflops": flops / (upd_ms * 1e-3) / 1e12,
+           "update_frac_of_fp64_peak": flops / (upd_ms * 1e-3) / 1e12 / PEAK_FP64_MFMA_TFLOPS,
+           "update_form": "n_w x n_w factors (m > n_w)" if m > n_w else "m x m factor",
+           "note": "flops of the form computed (kalman_flops in bench.py: m > n_w: 2 n_w^2 m + 8/3 n_w^3, else "
+                   "4 n_w^2 m + 4 m^2 n_w + m^3/3); its two rocSOLVER Cholesky factorisations of n_w rows are "
+                   "latency-bound panel kernels, most of the time; "
+                   f"peak = MI355X FP64 matrix {PEAK_FP64_MFMA_TFLOPS} TFLOP/s; HIP events over {reps} calls"}
+    if not args.no_cpu_baseline:
+        # the same update in numpy on the host, the reference's explicit inverse (one call)
+        P0 = kf.P.cpu().numpy().T.copy()
+        Hn = H.cpu().numpy().T.copy()         # n_w x m
+        xn = xi.cpu().numpy()
+        t0 = time.perf_counter()
+        X = P0 @ Hn
+        A = Hn.T @ X + (1.0 / kf.eta) * np.eye(m)
+        K = X @ np.linalg.inv(A)
+        P1 = P0 - K @ X.T + kf.q * np.eye(n_w)
+        W1 = K @ xn
+        out["cpu_numpy_force_update_ms"] = (time.perf_counter() - t0) * 1e3
+        out["cpu_threads"] = _oracle_threads()
+        del P1, W1
+    return out
+
+
 def phase_table(timers):
     """{phase: (total ms, launch groups)} from HIP-event pairs."""
     out = {}
@@ -361,6 +451,10 @@ def main() -> int:
     ap.add_argument("--no-graph", action="store_true", help="launch every step from the host (no HIP graphs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--kalman-updates", type=int, default=3,
+                    help="timed calls of the Kalman-filter leg (force Jacobian, force update, energy update); 0: skip")
+    ap.add_argument("--kalman-only", action="store_true",
+                    help="run only the Kalman-filter leg and print its JSON (for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, launch 3 plain FP64 copy kernels of MIB MiB each (HBM counter calibration)")
     args = ap.parse_args()
@@ -376,6 +470,9 @@ def main() -> int:
     from pantea_amd.units import units
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.kalman_only:
+        print(json.dumps({"kalman": run_kalman(args)}), flush=True)
+        return 0
     mev = units.TO_MILLI_ELECTRON_VOLT
     t_setup = time.perf_counter()
     s, teacher = bench_models.water_teacher(precision="mixed", molecules=args.molecules, box=args.box)
@@ -469,7 +566,8 @@ def main() -> int:
     pot_of = student if is_student else teacher
     if is_student:
         nb_s, tr_s = neighbour_stats(s, student.r_cutoff)
-        fwd_ops, adj_ops = algorithmic_ops(nb_s, tr_s, 8, 6, quartic=False)
+        # the student's tables are completed to the four-term class: quartic adjoint
+        fwd_ops, adj_ops = algorithmic_ops(nb_s, tr_s, 8, 6, quartic=True)
     else:
         fwd_ops, adj_ops = algorithmic_ops(n_nb, n_trip, 8, 12, quartic=True)
     mode = 1 if ph == "acsf_adjoint" else 0
@@ -517,6 +615,11 @@ def main() -> int:
             _log(f"[bench] cpu baseline: {cpu['value']:.4g} atom-steps/s on {cpu['cores']} threads")
         except Exception as exc:  # the baseline is a report, not the measurement
             _log(f"[bench] cpu baseline failed: {exc!r}")
+
+    kalman = None
+    if args.kalman_updates > 0:
+        kalman = run_kalman(args)
+        _log(f"[bench] Kalman filter: {kalman}")
 
     if args.copy_probe > 0:
         # y = 1.0 * x streams exactly 2 * MIB MiB through HBM per launch
@@ -587,6 +690,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_baseline_legs": cpu_legs,
+        "kalman": kalman,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -349,7 +349,10 @@ int pt_kalman_residual(int n, const double *pred, const double *ref, double ref_
  * P: (device, in/out) n_w x n_w FP64; H: (device) m columns of n_w (leading
  * dimension ldh), column o = gradient of observation o; xi: m; W: n_w.
  * A is factorised by Cholesky (rocSOLVER dpotrf/dpotrs) instead of being
- * inverted; info (device int) receives dpotrf's status (0 = success).
+ * inverted; with m > n_w the same update is computed from the n_w x n_w
+ * factors of P and of S = G G^T + r I, G = L^T H (push-through identity:
+ * P - K X^T = r L S^-1 L^T).  info (device int) receives the first nonzero
+ * dpotrf status (0 = success).
  * ws: pt_kalman_workspace_bytes(n_w, m) bytes of device memory. */
 size_t pt_kalman_workspace_bytes(int n_w, int m);
 int pt_kalman_update(int n_w, int m, double *P, const double *H, long long ldh, const double *xi,

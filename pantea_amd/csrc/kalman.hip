@@ -17,7 +17,23 @@
 // noise; the schedules of eta, q and lambda are host scalars.  The
 // reference forms inv(A) and multiplies; the Cholesky solve gives the same
 // K (A is symmetric positive definite: H^T P H is semi-definite and r > 0)
-// without forming the inverse.  All matrices are column-major FP64:
+// without forming the inverse.
+//
+// With more observations than weights (m > n_w: the force update of a large
+// structure) the same update is computed through n_w x n_w factors instead
+// of m x m ones.  With P = L L^T and G = L^T H, A = G^T G + r I, and by the
+// push-through identity K = X A^-1 = L S^-1 G with S = G G^T + r I (n_w x n_w),
+// so that P - K X^T = r L S^-1 L^T.  With S = Ls Ls^T and M = L Ls^-T:
+//
+//   L = chol(P)                                  rocSOLVER dpotrf (n_w)
+//   G = L^T H                        n_w x m     rocBLAS dtrmm
+//   S = G G^T + r I,  Ls = chol(S)   n_w x n_w   rocBLAS dsyrk + dpotrf
+//   [M; y^T] = [L; (G xi)^T] Ls^-T               dgemv + dtrsm (one solve: y = Ls^-1 G xi)
+//   W = W + M y  (= L S^-1 G xi)                 dgemv
+//   P = c r M M^T + q I                          dsyrk + k_symm_lower
+//
+// about 2 n_w^2 m + 8/3 n_w^3 flops against 4 n_w^2 m + 4 m^2 n_w + m^3 / 3,
+// and factorisations of n_w rows instead of m.  All matrices are column-major FP64:
 // P (n_w x n_w, ld n_w) and H (n_w x m, ld ldh) -- H column o is the
 // gradient of observation o, i.e. a row-major (m x n_w) Jacobian.
 #include <mutex>
@@ -34,6 +50,29 @@ namespace {
 __global__ void k_add_diag(int n, double* __restrict__ A, long long lda, double v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) A[(size_t)i * lda + i] += v;
+}
+
+// M ((n + 1) x n, ld ldm > n) = the lower triangle of A (zeros above) with g^T as its last row
+__global__ void k_lower_copy(int n, const double* __restrict__ A, long long lda, const double* __restrict__ g,
+                             double* __restrict__ M, long long ldm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;   // row
+    const int j = blockIdx.y;                             // column
+    if (i < n) M[(size_t)j * ldm + i] = i >= j ? A[(size_t)j * lda + i] : 0.0;
+    else if (i == n) M[(size_t)j * ldm + n] = g[j];
+}
+
+// A = its lower triangle mirrored into the upper one, plus v on the diagonal
+__global__ void k_symm_lower(int n, double* __restrict__ A, long long lda, double v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;   // row
+    const int j = blockIdx.y;                             // column
+    if (i >= n) return;
+    if (i > j) A[(size_t)i * lda + j] = A[(size_t)j * lda + i];
+    else if (i == j) A[(size_t)j * lda + i] += v;
+}
+
+// the first of two factorisations' info codes that is not zero
+__global__ void k_merge_info(int* __restrict__ info, const int* __restrict__ info2) {
+    if (threadIdx.x == 0 && *info == 0) *info = *info2;
 }
 
 // B (cols x rows, ld ldb) = A^T, A rows x cols (ld lda), 32 x 32 tiles through LDS
@@ -96,6 +135,9 @@ rocblas_handle handle_for_current_device(int* rc) {
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// the n_w x n_w form when the observations outnumber the weights
+bool factored_form(int n_w, int m) { return m > n_w; }
+
 }  // namespace
 }  // namespace pt
 
@@ -104,6 +146,9 @@ using namespace pt;
 extern "C" size_t pt_kalman_workspace_bytes(int n_w, int m) {
     if (n_w < 1 || m < 1) return 0;
     const size_t nw = (size_t)n_w, mm = (size_t)m;
+    if (factored_form(n_w, m))   // G, S, [M; y^T], g, info
+        return align256(8 * nw * mm) + align256(8 * nw * nw) + align256(8 * (nw + 1) * nw) + align256(8 * nw) +
+               align256(sizeof(int));
     return align256(8 * nw * mm) + align256(8 * mm * mm) + align256(8 * mm * nw);
 }
 
@@ -132,8 +177,8 @@ extern "C" int pt_kalman_update(int n_w, int m, double* P, const double* H, long
     PT_REQUIRE(ldh >= n_w, "pt_kalman_update: ldh < n_w");
     PT_REQUIRE(r > 0.0, "pt_kalman_update: the measurement noise must be positive");
     PT_REQUIRE(ws_bytes >= pt_kalman_workspace_bytes(n_w, m), "pt_kalman_update: workspace too small");
-    PT_REQUIRE((size_t)n_w * n_w < (1ull << 31) && (size_t)m * m < (1ull << 31) && (size_t)n_w * m < (1ull << 31) &&
-                   ldh < (1LL << 31),
+    PT_REQUIRE((size_t)n_w * n_w < (1ull << 31) && (factored_form(n_w, m) || (size_t)m * m < (1ull << 31)) &&
+                   (size_t)n_w * m < (1ull << 31) && ldh < (1LL << 31),
                "pt_kalman_update: matrices beyond 32-bit BLAS indexing");
     int rc = PT_OK;
     rocblas_handle h = handle_for_current_device(&rc);
@@ -141,10 +186,51 @@ extern "C" int pt_kalman_update(int n_w, int m, double* P, const double* H, long
     hipStream_t st = (hipStream_t)stream;
     PT_BLAS(rocblas_set_stream(h, st), "rocblas_set_stream");
     char* base = (char*)ws;
+    const double one = 1.0, zero = 0.0, mc = -c;
+    if (factored_form(n_w, m)) {
+        const size_t sz_g = align256(8 * (size_t)n_w * m), sz_s = align256(8 * (size_t)n_w * n_w);
+        const size_t sz_m = align256(8 * ((size_t)n_w + 1) * n_w);
+        const int ldm = n_w + 1;
+        double* G = (double*)base;
+        double* S = (double*)(base + sz_g);
+        double* M = (double*)(base + sz_g + sz_s);
+        double* g = (double*)(base + sz_g + sz_s + sz_m);
+        int* info2 = (int*)(base + sz_g + sz_s + sz_m + align256(8 * (size_t)n_w));
+        const dim3 grid2((n_w + 256) / 256, n_w);   // rows 0 .. n_w
+        // L = chol(P), lower triangle in place
+        PT_BLAS(rocsolver_dpotrf(h, rocblas_fill_lower, n_w, P, n_w, info), "dpotrf P");
+        // G = L^T H
+        PT_BLAS(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
+                              rocblas_diagonal_non_unit, n_w, m, &one, P, n_w, H, (rocblas_int)ldh, G, n_w),
+                "dtrmm L^T H");
+        // S = G G^T + r I = Ls Ls^T
+        PT_BLAS(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, n_w, m, &one, G, n_w, &zero, S, n_w),
+                "dsyrk G G^T");
+        hipLaunchKernelGGL(k_add_diag, dim3((n_w + 255) / 256), dim3(256), 0, st, n_w, S, (long long)n_w, r);
+        if (int e = launch_check("k_add_diag")) return e;
+        PT_BLAS(rocsolver_dpotrf(h, rocblas_fill_lower, n_w, S, n_w, info2), "dpotrf S");
+        hipLaunchKernelGGL(k_merge_info, dim3(1), dim3(64), 0, st, info, (const int*)info2);
+        if (int e = launch_check("k_merge_info")) return e;
+        // [M; y^T] = [L; (G xi)^T] Ls^-T: M = L Ls^-T and y = Ls^-1 G xi in one solve
+        PT_BLAS(rocblas_dgemv(h, rocblas_operation_none, n_w, m, &one, G, n_w, xi, 1, &zero, g, 1), "dgemv G xi");
+        hipLaunchKernelGGL(k_lower_copy, grid2, dim3(256), 0, st, n_w, (const double*)P, (long long)n_w,
+                           (const double*)g, M, (long long)ldm);
+        if (int e = launch_check("k_lower_copy")) return e;
+        PT_BLAS(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                              rocblas_diagonal_non_unit, n_w + 1, n_w, &one, S, n_w, M, ldm),
+                "dtrsm [L; g^T] Ls^-T");
+        // W += M y (= L S^-1 G xi), then P = c r M M^T + q I
+        PT_BLAS(rocblas_dgemv(h, rocblas_operation_none, n_w, n_w, &one, M, ldm, M + n_w, ldm, &one, W, 1),
+                "dgemv M y");
+        const double cr = c * r;
+        PT_BLAS(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, n_w, n_w, &cr, M, ldm, &zero, P, n_w),
+                "dsyrk M M^T");
+        hipLaunchKernelGGL(k_symm_lower, grid2, dim3(256), 0, st, n_w, P, (long long)n_w, q);
+        return launch_check("k_symm_lower");
+    }
     double* X = (double*)base;
     double* A = (double*)(base + align256(8 * (size_t)n_w * m));
     double* Y = (double*)(base + align256(8 * (size_t)n_w * m) + align256(8 * (size_t)m * m));
-    const double one = 1.0, zero = 0.0, mc = -c;
     // X = P H
     PT_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n_w, m, n_w, &one, P, n_w, H,
                           (rocblas_int)ldh, &zero, X, n_w),

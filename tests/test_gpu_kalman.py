@@ -5,7 +5,8 @@
   reference's h2o fixture (potential h2o.json + weights, structures of
   h2o.data): FP64, 1e-10 of the largest entry;
 * pt_kalman_update (rocBLAS + rocSOLVER Cholesky) against the oracle's
-  explicit-inverse update, filter types 0 and 1, m = 1 to 1,800;
+  explicit-inverse update, filter types 0 and 1, m = 1 to 1,800: m <= n_w
+  takes the m x m factorisation, m > n_w the n_w x n_w one;
 * KalmanFilter.fit over h2o.data (force fraction 0.5, three epochs, types 0
   and 1) against the oracle's fit loop with the reference's random
   generators: weights and losses;
@@ -82,7 +83,7 @@ def test_jacobians_match_complex_step():
 
 
 @pytest.mark.parametrize("kalman_type", [0, 1])
-@pytest.mark.parametrize("n_w,m", [(40, 1), (40, 36), (600, 1800)])
+@pytest.mark.parametrize("n_w,m", [(40, 1), (40, 36), (40, 40), (40, 90), (600, 1800)])
 def test_update_matches_oracle(kalman_type, n_w, m):
     from pantea_amd import _lib
 

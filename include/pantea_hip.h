@@ -168,13 +168,16 @@ int pt_nbr_build(int n, const double *pos, int pos_stride, const int *types, int
  *    [2] > cap means a build overflowed the arrays (the list is then
  *    incomplete); the caller checks [1] against the max_row it passes to the
  *    descriptor kernels (a longer row turns that centre's outputs into NaN);
- *  - rows longer than 256 entries take the second pass, decided on the device.
+ *  - rows longer than 256 entries take a second pass, decided on the device;
+ *    it is enqueued only when row_limit (the longest row the caller's kernels
+ *    read, i.e. the max_row it passes them) exceeds 256: a list with a row
+ *    over row_limit is unusable either way, and totals[1] reports it.
  * No synchronisation and no host read: every launch is enqueued whatever the
  * data, so the sequence can be replayed or captured. */
 int pt_nbr_build_async(int n, const double *pos, int pos_stride, const int *types, int n_types,
                        const double *box, double rc, int *perm, double *pos4, long long *row_off,
                        int *nbr, int *rev, long long cap, const int *gate, long long *totals,
-                       void *ws, size_t ws_bytes, pt_stream_t stream);
+                       int row_limit, void *ws, size_t ws_bytes, pt_stream_t stream);
 
 /* Mark the closed neighbourhoods of n_sel atoms: flags[s] = 1 and
  * flags[j] = 1 for every j in the row of s (flags: device, n entries, not

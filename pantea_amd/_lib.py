@@ -102,7 +102,7 @@ _SIGS = {
     "pt_nbr_build": (_c_int, [_c_int, _vp, _c_int, _vp, _c_int, _vp, _c_dbl, _vp, _vp, _vp, _vp, _vp,
                               _c_ll, ctypes.POINTER(_c_ll), ctypes.POINTER(_c_int), _vp, _c_size, _vp]),
     "pt_nbr_build_async": (_c_int, [_c_int, _vp, _c_int, _vp, _c_int, _vp, _c_dbl, _vp, _vp, _vp, _vp, _vp,
-                                    _c_ll, _vp, _vp, _vp, _c_size, _vp]),
+                                    _c_ll, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     "pt_nbr_mark_rows": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
     "pt_skin_check": (_c_int, [_c_int, _vp, _vp, _vp, _c_dbl, _vp, _vp, _vp]),
     "pt_copy_gated": (_c_int, [_c_ll, _c_int, _vp, _vp, _vp, _vp]),

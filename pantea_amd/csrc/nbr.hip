@@ -132,10 +132,14 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const int* __restri
 
 // Exclusive scan of m ints (up to a few 1e5) in place by one block: each
 // thread scans a contiguous chunk of ceil(m / 1024), the chunk totals are
-// scanned across the block (wave shuffles, then the 16 wave totals).
-__global__ __launch_bounds__(1024) void k_scan_small(const int* __restrict__ gate, int m, int* data) {
-    if (gated_off(gate)) return;
-    __shared__ int wtot[1024 / kWave];
+// scanned across the block (wave shuffles, then the 16 wave totals).  Up to
+// kScanLds entries go through LDS, loaded and stored coalesced (the chunks'
+// strided reads from HBM kept one load in flight per thread: 12.9 us for the
+// 12,288 radix counters of 98,304 atoms).
+constexpr int kScanLds = 16384;
+
+template <bool LDS>
+__device__ __forceinline__ void scan_chunks(int m, int* data, int* wtot) {
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x >> 6;
     const int per = (m + 1023) / 1024;
@@ -167,6 +171,22 @@ __global__ __launch_bounds__(1024) void k_scan_small(const int* __restrict__ gat
         const int v = data[k];
         data[k] = run;
         run += v;
+    }
+    (void)LDS;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_small(const int* __restrict__ gate, int m, int* data) {
+    if (gated_off(gate)) return;
+    __shared__ int wtot[1024 / kWave];
+    __shared__ int buf[kScanLds];
+    if (m <= kScanLds) {
+        for (int k = threadIdx.x; k < m; k += 1024) buf[k] = data[k];
+        __syncthreads();
+        scan_chunks<true>(m, buf, wtot);
+        __syncthreads();
+        for (int k = threadIdx.x; k < m; k += 1024) data[k] = buf[k];
+    } else {
+        scan_chunks<false>(m, data, wtot);
     }
 }
 
@@ -584,8 +604,10 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const int* __restrict__ gat
     if (threadIdx.x == 0) { psum[blockIdx.x] = ss[0]; pmax[blockIdx.x] = sm[0]; }
 }
 
+// totals (when set): the running totals of the asynchronous build (see pt_nbr_build_async)
 __global__ __launch_bounds__(1024) void k_scan_top(const int* __restrict__ gate, int nb, long long* psum,
-                                                   const int* pmax, long long* total_and_max) {
+                                                   const int* pmax, long long* total_and_max,
+                                                   long long* __restrict__ totals) {
     if (gated_off(gate)) return;
     __shared__ long long buf[1024];
     __shared__ long long carry;
@@ -616,7 +638,16 @@ __global__ __launch_bounds__(1024) void k_scan_top(const int* __restrict__ gate,
         if ((int)threadIdx.x < o) mx[threadIdx.x] = max(mx[threadIdx.x], mx[threadIdx.x + o]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) { total_and_max[0] = carry; total_and_max[1] = mx[0]; }
+    if (threadIdx.x == 0) {
+        total_and_max[0] = carry;
+        total_and_max[1] = mx[0];
+        if (totals) {
+            totals[0] = carry;
+            totals[1] = mx[0] > totals[1] ? mx[0] : totals[1];
+            totals[2] = carry > totals[2] ? carry : totals[2];
+            totals[3] += 1;
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_scan_down(const int* __restrict__ gate, int n, const int* counts,
@@ -867,17 +898,6 @@ namespace {
 
 // totals of a gated build: [0] pairs of the last build, [1] largest row and
 // [2] largest pair count since the caller cleared them, [3] builds done
-__global__ void k_totals(const int* __restrict__ gate, const long long* __restrict__ tm,
-                         long long* __restrict__ totals) {
-    if (gated_off(gate)) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        totals[0] = tm[0];
-        totals[1] = tm[1] > totals[1] ? tm[1] : totals[1];
-        totals[2] = tm[0] > totals[2] ? tm[0] : totals[2];
-        totals[3] += 1;
-    }
-}
-
 struct BuildOut {
     int* perm;
     double* pos4;
@@ -893,7 +913,7 @@ struct BuildOut {
 // to dev_totals.
 int build_impl(int n, const double* pos, int pos_stride, const int* types, int n_types, const double* box,
                double rc, const BuildOut& o, void* ws, size_t ws_bytes, const int* gate,
-               long long* host_totals, long long* dev_totals, hipStream_t stream) {
+               long long* host_totals, long long* dev_totals, int row_limit, hipStream_t stream) {
     WsLayout L = ws_layout(n, n_types);
     PT_REQUIRE(ws_bytes >= L.total, "pt_nbr_build: workspace too small (%zu < %zu)", ws_bytes, L.total);
     char* w = (char*)ws;
@@ -1010,7 +1030,7 @@ int build_impl(int n, const double* pos, int pos_stride, const int* types, int n
     const int nsb = (n + kScanBlock - 1) / kScanBlock;
     long long* tot_max = (long long*)misc;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(256), 0, stream, gate, n, counts, psum, pmax);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, gate, nsb, psum, pmax, tot_max);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, gate, nsb, psum, pmax, tot_max, dev_totals);
     hipLaunchKernelGGL(k_scan_down, dim3(nsb), dim3(256), 0, stream, gate, n, counts, psum, tot_max, o.row_off);
     if (int rcode = launch_check("row offsets")) return rcode;
     // the compaction checks the totals itself, so it is queued before the host
@@ -1022,14 +1042,14 @@ int build_impl(int n, const double* pos, int pos_stride, const int* types, int n
         if (int rcode = launch_check("k_compact_rev")) return rcode;
     }
     if (!sync) {
-        // rows longer than the stage: the second pass decides on the device whether it runs
-        if (o.nbr) {
+        // rows longer than the stage: the second pass decides on the device whether it runs; it is
+        // not enqueued when the caller reads no row that long (such a list is reported in totals[1])
+        if (o.nbr && row_limit > kStageW) {
             launch_cell(true, (const long long*)tot_max, (const long long*)o.row_off, o.nbr, nullptr);
             if (o.rev)
                 hipLaunchKernelGGL(k_rev, dim3(nwb), dim3(256), 0, stream, gate, (const long long*)tot_max, o.cap,
                                    n, (const long long*)o.row_off, (const int*)o.nbr, o.rev);
         }
-        hipLaunchKernelGGL(k_totals, dim3(1), dim3(64), 0, stream, gate, (const long long*)tot_max, dev_totals);
         return launch_check("gated build tail");
     }
     // synchronous: the totals are read through the caller's stack (a pageable
@@ -1074,7 +1094,7 @@ extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int*
     BuildOut o{perm, pos4, row_off, nbr, rev, cap};
     long long tm[2] = {0, 0};
     const int rc_ = build_impl(n, pos, pos_stride, types, n_types, box, rc, o, ws, ws_bytes, nullptr, tm,
-                               nullptr, (hipStream_t)stream_);
+                               nullptr, INT_MAX, (hipStream_t)stream_);
     if (rc_ == PT_OK || rc_ == PT_ERR_CAPACITY) {
         *n_pairs = tm[0];
         *max_row = (int)tm[1];
@@ -1085,7 +1105,7 @@ extern "C" int pt_nbr_build(int n, const double* pos, int pos_stride, const int*
 extern "C" int pt_nbr_build_async(int n, const double* pos, int pos_stride, const int* types, int n_types,
                                   const double* box, double rc, int* perm, double* pos4, long long* row_off,
                                   int* nbr, int* rev, long long cap, const int* gate, long long* totals,
-                                  void* ws, size_t ws_bytes, pt_stream_t stream_) {
+                                  int row_limit, void* ws, size_t ws_bytes, pt_stream_t stream_) {
     PT_REQUIRE(n >= 1, "pt_nbr_build_async: n must be >= 1 (got %d)", n);
     PT_REQUIRE(n_types >= 1 && n_types <= kMaxTypes, "pt_nbr_build_async: n_types must be in [1, %d]",
                kMaxTypes);
@@ -1097,7 +1117,7 @@ extern "C" int pt_nbr_build_async(int n, const double* pos, int pos_stride, cons
     PT_REQUIRE((const void*)pos != (const void*)pos4, "pt_nbr_build_async: pos and pos4 must not alias");
     BuildOut o{perm, pos4, row_off, nbr, rev, cap};
     return build_impl(n, pos, pos_stride, types, n_types, box, rc, o, ws, ws_bytes, gate, nullptr, totals,
-                      (hipStream_t)stream_);
+                      row_limit, (hipStream_t)stream_);
 }
 
 extern "C" int pt_nbr_mark_rows(int n_sel, const int* sel, const long long* row_off, const int* nbr,

@@ -341,22 +341,27 @@ class _DeviceMD:
         box = host_box(self.box)
         box_p = ctypes.cast(box, ctypes.c_void_p)
         g = self.flag.data_ptr()
-        _lib.check(lib.pt_skin_check(self.n, nl.pos4.data_ptr(), self.ref_pos.data_ptr(), box_p, self.skin,
-                                     self.disp.data_ptr(), g, st), "pt_skin_check")
+        if self.skin > 0.0:
+            _lib.check(lib.pt_skin_check(self.n, nl.pos4.data_ptr(), self.ref_pos.data_ptr(), box_p, self.skin,
+                                         self.disp.data_ptr(), g, st), "pt_skin_check")
+        # without a skin the flag keeps the 1 it was created with: every step rebuilds, and
+        # neither the displacement check nor the reference positions are needed
         _lib.check(lib.pt_copy_gated(4 * self.n, 8, nl.pos4.data_ptr(), self.pos_in.data_ptr(), g, st),
                    "pt_copy_gated")
         _lib.check(lib.pt_nbr_build_async(self.n, self.pos_in.data_ptr(), 4, self.types.data_ptr(), nl.n_types,
                                           box_p, nl.r_cutoff, nl.perm.data_ptr(), nl.pos4.data_ptr(),
                                           nl.row_off.data_ptr(), nl.nbr.data_ptr(), nl.rev.data_ptr(), nl.cap, g,
-                                          self.totals.data_ptr(), nl.ws.data_ptr(), nl.ws.numel(), st),
+                                          self.totals.data_ptr(), int(self.row_bound), nl.ws.data_ptr(),
+                                          nl.ws.numel(), st),
                    "pt_nbr_build_async")
         arrays = (_lib.PermuteArray * 6)(*[_lib.PermuteArray(b.data_ptr(), w, wb) for b, w, wb in (
             (self.vel, 3, 8), (self.force, 3, 8), (self.mass, 1, 8), (self.inv_mass, 1, 8), (self.orig, 1, 4),
             (self.types, 1, 4))])
         _lib.check(lib.pt_permute_state_gated(self.n, nl.perm.data_ptr(), 6, arrays, self.tmp_state.data_ptr(),
                                               self.tmp_state.numel(), g, st), "pt_permute_state_gated")
-        _lib.check(lib.pt_copy_gated(4 * self.n, 8, nl.pos4.data_ptr(), self.ref_pos.data_ptr(), g, st),
-                   "pt_copy_gated")
+        if self.skin > 0.0:
+            _lib.check(lib.pt_copy_gated(4 * self.n, 8, nl.pos4.data_ptr(), self.ref_pos.data_ptr(), g, st),
+                       "pt_copy_gated")
         # the descriptor kernels stage rows up to the bound; buffers are sized for the capacity
         nl.max_row = self.row_bound
         nl.n_pairs = nl.cap

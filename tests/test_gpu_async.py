@@ -32,7 +32,7 @@ def _water(n_mol, L, seed=2023):
     return Structure.from_dict(water.water_box(n_mol, L, seed=seed))
 
 
-def _sync_and_async(s, rc, gate_value, cap=None):
+def _sync_and_async(s, rc, gate_value, cap=None, row_limit=1 << 30):
     from pantea_amd import _lib
     from pantea_amd.atoms import host_box
     from pantea_amd.engine import NeighborList
@@ -55,7 +55,7 @@ def _sync_and_async(s, rc, gate_value, cap=None):
                                       s.element_map.n_types, ctypes.cast(box, ctypes.c_void_p), rc,
                                       out["perm"].data_ptr(), out["pos4"].data_ptr(), out["row_off"].data_ptr(),
                                       out["nbr"].data_ptr(), out["rev"].data_ptr(), cap, gate.data_ptr(),
-                                      totals.data_ptr(), ws.data_ptr(), ws.numel(), _lib.stream_ptr()),
+                                      totals.data_ptr(), row_limit, ws.data_ptr(), ws.numel(), _lib.stream_ptr()),
                "pt_nbr_build_async")
     torch.cuda.synchronize()
     return nl, out, totals.cpu().numpy()
@@ -83,6 +83,22 @@ def test_gated_build_equals_sync_build(case):
     assert torch.equal(out["nbr"][:m], nl.nbr[:m]) and torch.equal(out["rev"][:m], nl.rev[:m])
     assert (out["nbr"][m:] == -7).all()
     assert tot.tolist() == [m, nl.max_row, m, 1]
+
+
+def test_rows_over_the_row_limit_are_reported_without_the_second_pass():
+    # the caller reads rows up to 200 entries: the long-row pass is not enqueued, the
+    # list stays unwritten and totals[1] tells the caller that it is unusable
+    from pantea_amd.atoms import Structure
+
+    rng = np.random.default_rng(3)
+    pos = np.concatenate([rng.normal(size=(400, 3)) * 1.2 + c for c in (5.0, 20.0)])
+    s = Structure.from_dict({"positions": np.mod(pos, 30.0), "elements": ["H"] * 800,
+                             "lattice": np.diag([30.0] * 3)})
+    nl, out, tot = _sync_and_async(s, 4.0, 1, row_limit=200)
+    assert nl.max_row > 256
+    assert tot.tolist() == [nl.n_pairs, nl.max_row, nl.n_pairs, 1] and tot[1] > 200
+    assert (out["nbr"] == -7).all() and (out["rev"] == -7).all()
+    assert torch.equal(out["row_off"], nl.row_off)
 
 
 def test_closed_gate_leaves_outputs_untouched():

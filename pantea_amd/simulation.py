@@ -21,9 +21,10 @@ two largest atom displacements since the last build sum to more than the
 skin (no pair can have moved by more); the descriptor kernels re-apply the
 exact mask (r <= rc, r > 0), so every step sees exactly the reference's
 neighbour set.  In a periodic box that decision is taken on the device
-(pt_skin_check) and the rebuild runs behind its flag (pt_nbr_build_async,
-pt_permute_rows_gated), so a step never waits for the GPU; the list's
-totals are verified whenever the host reads the state.
+(pt_skin_check; without a skin every step rebuilds) and the rebuild runs
+behind its flag (pt_nbr_build_async, pt_permute_state_gated), so a step
+never waits for the GPU; the list's totals are verified whenever the host
+reads the state.
 """
 
 from __future__ import annotations

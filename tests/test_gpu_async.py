@@ -207,16 +207,20 @@ def _trajectory(s, pot, precision, sync_free, steps=60, row_bound=None, thermost
     return x, v, md.n_builds, md.redos
 
 
+@pytest.mark.parametrize("skin", [1.0, 0.0])
 @pytest.mark.parametrize("precision", ["float64", "mixed"])
-def test_gated_md_equals_synchronous_md(precision):
+def test_gated_md_equals_synchronous_md(precision, skin):
+    # skin 0: no displacement check, a rebuild every step
     from pantea_amd.bench_models import water_nnp
 
     s = _water(128, 29.56)
     pot = water_nnp(s, hidden=(25, 25), seed=2023)
-    x0, v0, b0, _ = _trajectory(s, pot, precision, False)
-    x1, v1, b1, _ = _trajectory(s, pot, precision, True)
-    print(f"{precision}: {b0} synchronous builds, {b1} gated builds", flush=True)
+    x0, v0, b0, _ = _trajectory(s, pot, precision, False, skin=skin)
+    x1, v1, b1, _ = _trajectory(s, pot, precision, True, skin=skin)
+    print(f"{precision}, skin {skin}: {b0} synchronous builds, {b1} gated builds", flush=True)
     assert b0 == b1 and b0 > 2
+    if skin == 0.0:
+        assert b0 == 61
     np.testing.assert_array_equal(x1, x0)
     np.testing.assert_array_equal(v1, v0)
 

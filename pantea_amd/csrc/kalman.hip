@@ -75,6 +75,74 @@ __global__ void k_merge_info(int* __restrict__ info, const int* __restrict__ inf
     if (threadIdx.x == 0 && *info == 0) *info = *info2;
 }
 
+__global__ void k_zero_int(int* __restrict__ p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
+// ---- Cholesky factorisation A = L L^T (lower, in place), right-looking in
+// blocks of kCholNb columns.  Each diagonal block is factorised by one
+// wavefront in LDS (lane i owns row i), which also writes the block's inverse;
+// the panel below is then L_21 = A_21 L_11^-T, a GEMM with that inverse, and
+// the trailing matrix takes A_22 -= L_21 L_21^T (rocBLAS dsyrk).  rocSOLVER's
+// dpotrf factorises its diagonal panels in a kernel of ~156 us each, most of
+// the time of a factorisation of a few thousand rows.
+constexpr int kCholNb = 64;
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u & 0xffffffffu), lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Lane l holds row l of the block in registers (r[c] = A(k0 + l, k0 + c)); the
+// column entries a step needs from other rows are read with v_readlane, so the
+// factorisation and the inverse run without LDS or barriers.
+__global__ __launch_bounds__(kCholNb) void k_chol_diag(int n, double* __restrict__ A, long long lda, int k0,
+                                                       double* __restrict__ Linv, int* __restrict__ info) {
+    const int l = threadIdx.x;
+    const int nb = min(kCholNb, n - k0);
+    double r[kCholNb];
+#pragma unroll
+    for (int c = 0; c < kCholNb; ++c)
+        r[c] = (l < nb && c < nb && l >= c) ? A[(size_t)(k0 + c) * lda + k0 + l] : 0.0;
+    int bad = 0;   // 1-based column of the first pivot that is not positive (the same in every lane)
+#pragma unroll
+    for (int j = 0; j < kCholNb; ++j) {
+        if (j < nb) {
+            const double ajj = readlane_f64(r[j], j);
+            if (!(ajj > 0.0) && bad == 0) bad = j + 1;
+            const double d = sqrt(ajj);
+            const double inv = 1.0 / d;
+            if (l == j) r[j] = d;
+            else if (l > j) r[j] *= inv;
+            // A(l, c) -= L(l, j) L(c, j) for j < c <= l
+#pragma unroll
+            for (int c = j + 1; c < kCholNb; ++c) {
+                const double lcj = readlane_f64(r[j], c);
+                if (l >= c) r[c] -= r[j] * lcj;
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < kCholNb; ++c)
+        if (l < nb && c < nb && l >= c) A[(size_t)(k0 + c) * lda + k0 + l] = r[c];
+    // inverse: lane l forms column l of X = L^-1 by forward substitution,
+    // x[i] = (delta(i, l) - sum_{k < i} L(i, k) x[k]) / L(i, i)
+    double x[kCholNb];
+#pragma unroll
+    for (int i = 0; i < kCholNb; ++i) {
+        double t = (i == l) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < i; ++k) t -= readlane_f64(r[k], i) * x[k];
+        x[i] = (i < nb) ? t / readlane_f64(r[i], i) : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < kCholNb; ++i) Linv[(size_t)l * kCholNb + i] = (l < nb) ? x[i] : 0.0;
+    if (l == 0 && bad != 0 && *info == 0) *info = k0 + bad;
+}
+
+
 // B (cols x rows, ld ldb) = A^T, A rows x cols (ld lda), 32 x 32 tiles through LDS
 __global__ void k_transpose(int rows, int cols, const double* __restrict__ A, long long lda,
                             double* __restrict__ B, long long ldb) {
@@ -138,27 +206,8 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 // the n_w x n_w form when the observations outnumber the weights
 bool factored_form(int n_w, int m) { return m > n_w; }
 
-}  // namespace
-}  // namespace pt
-
-using namespace pt;
-
-extern "C" size_t pt_kalman_workspace_bytes(int n_w, int m) {
-    if (n_w < 1 || m < 1) return 0;
-    const size_t nw = (size_t)n_w, mm = (size_t)m;
-    if (factored_form(n_w, m))   // G, S, [M; y^T], g, info
-        return align256(8 * nw * mm) + align256(8 * nw * nw) + align256(8 * (nw + 1) * nw) + align256(8 * nw) +
-               align256(sizeof(int));
-    return align256(8 * nw * mm) + align256(8 * mm * mm) + align256(8 * mm * nw);
-}
-
-extern "C" int pt_kalman_residual(int n, const double* pred, const double* ref, double ref_scalar, double scale,
-                                  double* xi, double* loss, pt_stream_t stream) {
-    PT_REQUIRE(n >= 1 && pred && xi && loss, "pt_kalman_residual: bad arguments");
-    hipLaunchKernelGGL(k_residual, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, pred, ref,
-                       ref_scalar, scale, xi, loss);
-    return launch_check("pt_kalman_residual");
-}
+// workspace of chol_lower for n rows: the block inverse and the panel
+size_t chol_ws_bytes(int n) { return align256(8 * (size_t)kCholNb * kCholNb) + align256(8 * (size_t)n * kCholNb); }
 
 #define PT_BLAS(call, what)                                                            \
     do {                                                                               \
@@ -168,6 +217,59 @@ extern "C" int pt_kalman_residual(int n, const double* pred, const double* ref, 
             return PT_ERR_HIP;                                                         \
         }                                                                              \
     } while (0)
+
+// A (n x n, lower triangle, ld lda) = L L^T in place; *info = 0 or the 1-based
+// column of the first pivot that is not positive (as dpotrf reports it).
+int chol_lower(rocblas_handle h, hipStream_t st, int n, double* A, int lda, char* ws, int* info) {
+    double* Linv = (double*)ws;
+    double* T = (double*)(ws + align256(8 * (size_t)kCholNb * kCholNb));
+    const double one = 1.0, zero = 0.0, mone = -1.0;
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, info);
+    if (int e = launch_check("k_zero_int")) return e;
+    for (int k0 = 0; k0 < n; k0 += kCholNb) {
+        const int nb = std::min(kCholNb, n - k0);
+        hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(kCholNb), 0, st, n, A, (long long)lda, k0, Linv, info);
+        if (int e = launch_check("k_chol_diag")) return e;
+        const int mk = n - k0 - nb;
+        if (mk <= 0) break;
+        double* A21 = A + (size_t)k0 * lda + k0 + nb;
+        double* A22 = A + (size_t)(k0 + nb) * lda + k0 + nb;
+        // T = A_21 L_11^-T, A_22 -= T T^T, A_21 = T
+        PT_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, mk, nb, nb, &one, A21, lda,
+                              Linv, kCholNb, &zero, T, mk),
+                "dgemm panel");
+        PT_BLAS(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, mk, nb, &mone, T, mk, &one, A22, lda),
+                "dsyrk trailing");
+        if (hipMemcpy2DAsync(A21, (size_t)lda * 8, T, (size_t)mk * 8, (size_t)mk * 8, nb, hipMemcpyDeviceToDevice,
+                             st) != hipSuccess) {
+            set_error("pt_kalman_update: panel copy failed");
+            return PT_ERR_HIP;
+        }
+    }
+    return PT_OK;
+}
+
+}  // namespace
+}  // namespace pt
+
+using namespace pt;
+
+extern "C" size_t pt_kalman_workspace_bytes(int n_w, int m) {
+    if (n_w < 1 || m < 1) return 0;
+    const size_t nw = (size_t)n_w, mm = (size_t)m;
+    if (factored_form(n_w, m))   // G, S, [M; y^T], g, info, Cholesky workspace
+        return align256(8 * nw * mm) + align256(8 * nw * nw) + align256(8 * (nw + 1) * nw) + align256(8 * nw) +
+               align256(sizeof(int)) + chol_ws_bytes(n_w);
+    return align256(8 * nw * mm) + align256(8 * mm * mm) + align256(8 * mm * nw) + chol_ws_bytes(m);
+}
+
+extern "C" int pt_kalman_residual(int n, const double* pred, const double* ref, double ref_scalar, double scale,
+                                  double* xi, double* loss, pt_stream_t stream) {
+    PT_REQUIRE(n >= 1 && pred && xi && loss, "pt_kalman_residual: bad arguments");
+    hipLaunchKernelGGL(k_residual, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, pred, ref,
+                       ref_scalar, scale, xi, loss);
+    return launch_check("pt_kalman_residual");
+}
 
 extern "C" int pt_kalman_update(int n_w, int m, double* P, const double* H, long long ldh, const double* xi,
                                 double* W, double r, double c, double q, void* ws, size_t ws_bytes, int* info,
@@ -196,9 +298,10 @@ extern "C" int pt_kalman_update(int n_w, int m, double* P, const double* H, long
         double* M = (double*)(base + sz_g + sz_s);
         double* g = (double*)(base + sz_g + sz_s + sz_m);
         int* info2 = (int*)(base + sz_g + sz_s + sz_m + align256(8 * (size_t)n_w));
+        char* cws = base + sz_g + sz_s + sz_m + align256(8 * (size_t)n_w) + align256(sizeof(int));
         const dim3 grid2((n_w + 256) / 256, n_w);   // rows 0 .. n_w
         // L = chol(P), lower triangle in place
-        PT_BLAS(rocsolver_dpotrf(h, rocblas_fill_lower, n_w, P, n_w, info), "dpotrf P");
+        if (int e = chol_lower(h, st, n_w, P, n_w, cws, info)) return e;
         // G = L^T H
         PT_BLAS(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
                               rocblas_diagonal_non_unit, n_w, m, &one, P, n_w, H, (rocblas_int)ldh, G, n_w),
@@ -208,7 +311,7 @@ extern "C" int pt_kalman_update(int n_w, int m, double* P, const double* H, long
                 "dsyrk G G^T");
         hipLaunchKernelGGL(k_add_diag, dim3((n_w + 255) / 256), dim3(256), 0, st, n_w, S, (long long)n_w, r);
         if (int e = launch_check("k_add_diag")) return e;
-        PT_BLAS(rocsolver_dpotrf(h, rocblas_fill_lower, n_w, S, n_w, info2), "dpotrf S");
+        if (int e = chol_lower(h, st, n_w, S, n_w, cws, info2)) return e;
         hipLaunchKernelGGL(k_merge_info, dim3(1), dim3(64), 0, st, info, (const int*)info2);
         if (int e = launch_check("k_merge_info")) return e;
         // [M; y^T] = [L; (G xi)^T] Ls^-T: M = L Ls^-T and y = Ls^-1 G xi in one solve
@@ -241,7 +344,9 @@ extern "C" int pt_kalman_update(int n_w, int m, double* P, const double* H, long
             "dgemm H^T X");
     hipLaunchKernelGGL(k_add_diag, dim3((m + 255) / 256), dim3(256), 0, st, m, A, (long long)m, r);
     if (int e = launch_check("k_add_diag")) return e;
-    PT_BLAS(rocsolver_dpotrf(h, rocblas_fill_lower, m, A, m, info), "dpotrf");
+    if (int e = chol_lower(h, st, m, A, m, base + align256(8 * (size_t)n_w * m) + align256(8 * (size_t)m * m) +
+                                              align256(8 * (size_t)m * n_w), info))
+        return e;
     // Y = X^T, then A Y = X^T
     hipLaunchKernelGGL(k_transpose, dim3((n_w + 31) / 32, (m + 31) / 32), dim3(256), 0, st, n_w, m, X,
                        (long long)n_w, Y, (long long)m);

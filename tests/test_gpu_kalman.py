@@ -119,6 +119,34 @@ def test_update_matches_oracle(kalman_type, n_w, m):
     assert np.abs(dW).max() <= 1e-10 * max(np.abs(st.W[:, 0]).max(), 1.0)
 
 
+@pytest.mark.parametrize("n_w,m,col", [(40, 90, 1), (130, 200, 70), (40, 36, 1)])
+def test_update_reports_a_failed_factorisation(n_w, m, col):
+    """An indefinite matrix to factorise sets info to the 1-based column of the first
+    pivot that is not positive, as dpotrf reports it (P itself in the n_w x n_w form,
+    A = H^T P H + r I in the m x m form)."""
+    from pantea_amd import _lib
+
+    lib = _lib.load()
+    rng = np.random.default_rng(7)
+    P = np.eye(n_w) * 50.0
+    if m > n_w:
+        P[col - 1, col - 1] = -1.0          # P is indefinite from this column on
+    else:
+        P = -np.eye(n_w)                    # H^T P H + r I is negative definite
+    H = rng.normal(size=(n_w, m)) * (0.02 if m > n_w else 1.0)
+    P_d = torch.as_tensor(P.T.copy(), device="cuda")
+    W_d = torch.zeros(n_w, dtype=torch.float64, device="cuda")
+    H_d = torch.as_tensor(H.T.copy(), device="cuda")
+    x_d = torch.as_tensor(rng.normal(size=m), device="cuda")
+    ws = torch.empty(lib.pt_kalman_workspace_bytes(n_w, m), dtype=torch.uint8, device="cuda")
+    info = torch.full((1,), -1, dtype=torch.int32, device="cuda")
+    _lib.check(lib.pt_kalman_update(n_w, m, P_d.data_ptr(), H_d.data_ptr(), n_w, x_d.data_ptr(), W_d.data_ptr(),
+                                    0.01, 1.0, 0.0, ws.data_ptr(), ws.numel(), info.data_ptr(), _lib.stream_ptr()),
+               "pt_kalman_update")
+    torch.cuda.synchronize()
+    assert int(info.item()) == col
+
+
 @pytest.mark.parametrize("kalman_type", [0, 1])
 def test_fit_matches_oracle_fit(kalman_type):
     from pantea_amd.kalman import NeuralNetworkPotentialTrainer

@@ -395,8 +395,8 @@ def run_kalman(args):
            "update_frac_of_fp64_peak": flops / (upd_ms * 1e-3) / 1e12 / PEAK_FP64_MFMA_TFLOPS,
            "update_form": "n_w x n_w factors (m > n_w)" if m > n_w else "m x m factor",
            "note": "flops of the form computed (kalman_flops in bench.py: m > n_w: 2 n_w^2 m + 8/3 n_w^3, else "
-                   "4 n_w^2 m + 4 m^2 n_w + m^3/3); its two rocSOLVER Cholesky factorisations of n_w rows are "
-                   "latency-bound panel kernels, most of the time; "
+                   "4 n_w^2 m + 4 m^2 n_w + m^3/3); its two Cholesky factorisations of n_w rows (blocked, one "
+                   "wavefront per 64 x 64 diagonal block) and their many small BLAS calls are latency-bound; "
                    f"peak = MI355X FP64 matrix {PEAK_FP64_MFMA_TFLOPS} TFLOP/s; HIP events over {reps} calls"}
     if not args.no_cpu_baseline:
         # the same update in numpy on the host, the reference's explicit inverse (one call)

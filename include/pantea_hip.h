@@ -351,7 +351,7 @@ int pt_kalman_residual(int n, const double *pred, const double *ref, double ref_
  * r = 1/eta (type 0) or lambda (type 1), c = 1 (type 0) or 1/lambda (type 1).
  * P: (device, in/out) n_w x n_w FP64; H: (device) m columns of n_w (leading
  * dimension ldh), column o = gradient of observation o; xi: m; W: n_w.
- * A is factorised by Cholesky (rocSOLVER dpotrf/dpotrs) instead of being
+ * A is factorised by Cholesky (blocked, then rocSOLVER dpotrs) instead of being
  * inverted; with m > n_w the same update is computed from the n_w x n_w
  * factors of P and of S = G G^T + r I, G = L^T H (push-through identity:
  * P - K X^T = r L S^-1 L^T).  info (device int) receives the first nonzero

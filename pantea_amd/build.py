@@ -37,7 +37,7 @@ HIPCC_FLAGS = [
 
 SOURCES = ["md.hip", "nbr.hip", "acsf.hip", "mlp.hip", "scaler.hip", "kalman.hip"]
 
-# the Kalman filter update calls rocBLAS (GEMM, GEMV) and rocSOLVER (Cholesky)
+# the Kalman filter update calls rocBLAS (GEMM, SYRK, TRMM, TRSM, GEMV) and rocSOLVER (dpotrs)
 LINK_FLAGS = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
 
 

@@ -7,7 +7,7 @@
 //
 //   X = P H                           n_w x m    rocBLAS dgemm
 //   A = H^T X + r I                   m x m      rocBLAS dgemm + k_add_diag
-//   A = L L^T                                    rocSOLVER dpotrf
+//   A = L L^T                                    chol_lower (below)
 //   Y = A^-1 X^T         (= K^T)      m x n_w    k_transpose + rocSOLVER dpotrs
 //   P = c (P - Y^T X^T) + q I                    rocBLAS dgemm + k_add_diag
 //   W = W + Y^T xi                               rocBLAS dgemv
@@ -25,7 +25,7 @@
 // push-through identity K = X A^-1 = L S^-1 G with S = G G^T + r I (n_w x n_w),
 // so that P - K X^T = r L S^-1 L^T.  With S = Ls Ls^T and M = L Ls^-T:
 //
-//   L = chol(P)                                  rocSOLVER dpotrf (n_w)
+//   L = chol(P)                                  chol_lower (n_w)
 //   G = L^T H                        n_w x m     rocBLAS dtrmm
 //   S = G G^T + r I,  Ls = chol(S)   n_w x n_w   rocBLAS dsyrk + dpotrf
 //   [M; y^T] = [L; (G xi)^T] Ls^-T               dgemv + dtrsm (one solve: y = Ls^-1 G xi)

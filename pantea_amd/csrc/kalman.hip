@@ -27,7 +27,7 @@
 //
 //   L = chol(P)                                  chol_lower (n_w)
 //   G = L^T H                        n_w x m     rocBLAS dtrmm
-//   S = G G^T + r I,  Ls = chol(S)   n_w x n_w   rocBLAS dsyrk + dpotrf
+//   S = G G^T + r I,  Ls = chol(S)   n_w x n_w   rocBLAS dsyrk + chol_lower
 //   [M; y^T] = [L; (G xi)^T] Ls^-T               dgemv + dtrsm (one solve: y = Ls^-1 G xi)
 //   W = W + M y  (= L S^-1 G xi)                 dgemv
 //   P = c r M M^T + q I                          dsyrk + k_symm_lower
@@ -81,7 +81,7 @@ __global__ void k_zero_int(int* __restrict__ p) {
 
 // ---- Cholesky factorisation A = L L^T (lower, in place), right-looking in
 // blocks of kCholNb columns.  Each diagonal block is factorised by one
-// wavefront in LDS (lane i owns row i), which also writes the block's inverse;
+// wavefront (lane i holds row i in registers), which also writes the block's inverse;
 // the panel below is then L_21 = A_21 L_11^-T, a GEMM with that inverse, and
 // the trailing matrix takes A_22 -= L_21 L_21^T (rocBLAS dsyrk).  rocSOLVER's
 // dpotrf factorises its diagonal panels in a kernel of ~156 us each, most of
@@ -141,7 +141,6 @@ __global__ __launch_bounds__(kCholNb) void k_chol_diag(int n, double* __restrict
     for (int i = 0; i < kCholNb; ++i) Linv[(size_t)l * kCholNb + i] = (l < nb) ? x[i] : 0.0;
     if (l == 0 && bad != 0 && *info == 0) *info = k0 + bad;
 }
-
 
 // B (cols x rows, ld ldb) = A^T, A rows x cols (ld lda), 32 x 32 tiles through LDS
 __global__ void k_transpose(int rows, int cols, const double* __restrict__ A, long long lda,

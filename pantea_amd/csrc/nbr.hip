@@ -476,12 +476,12 @@ __global__ __launch_bounds__(THREADS) void k_nbr_cell(const int* __restrict__ ga
     const int nr = s_nr;
     const int total = s_rpre[nr];
     const bool staged = total <= CAND;
-    // With at least 4 cells per dimension the cells of one run (x-consecutive ids)
-    // share one periodic image, and a pair within the cutoff is always seen in the
-    // image of the stencil cell that holds it (cells are at least rc wide): the
-    // staged candidates are then stored at that image and the distance test needs
-    // no per-pair image.  A pair seen in another image is farther apart than its
-    // minimum image, i.e. beyond the cutoff either way.
+    // Every run keeps to one periodic image (runs are split where it changes).  With
+    // at least 4 cells per dimension the 27 stencil cells are distinct and at least
+    // rc wide, so a pair within the cutoff is always seen in the image of the
+    // stencil cell that holds it: the staged candidates are stored at that image and
+    // the distance test needs no per-pair image.  A pair seen in another image is
+    // farther apart than its minimum image, i.e. beyond the cutoff either way.
     const bool direct = staged && (!box.periodic || (g.nc[0] >= 4 && g.nc[1] >= 4 && g.nc[2] >= 4));
     const float3 bf = make_float3((float)box.L[0], (float)box.L[1], (float)box.L[2]);
     if (staged) {
@@ -604,7 +604,9 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const int* __restrict__ gat
     if (threadIdx.x == 0) { psum[blockIdx.x] = ss[0]; pmax[blockIdx.x] = sm[0]; }
 }
 
-// totals (when set): the running totals of the asynchronous build (see pt_nbr_build_async)
+// totals (when set): the running totals of an asynchronous build: [0] pairs of the last
+// build, [1] largest row and [2] largest pair count since the caller cleared them,
+// [3] builds done
 __global__ __launch_bounds__(1024) void k_scan_top(const int* __restrict__ gate, int nb, long long* psum,
                                                    const int* pmax, long long* total_and_max,
                                                    long long* __restrict__ totals) {
@@ -896,8 +898,6 @@ extern "C" size_t pt_nbr_workspace_bytes(int n, int n_types) {
 namespace pt {
 namespace {
 
-// totals of a gated build: [0] pairs of the last build, [1] largest row and
-// [2] largest pair count since the caller cleared them, [3] builds done
 struct BuildOut {
     int* perm;
     double* pos4;

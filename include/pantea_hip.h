@@ -45,7 +45,7 @@ enum {
 enum { PT_F64 = 0, PT_F32 = 1 };
 
 /* Descriptor kernel variants.  The precision argument of pt_acsf_forward /
- * pt_acsf_adjoint may carry (kind << PT_ACSF_KIND_SHIFT), kind from
+ * pt_acsf_adjoint / pt_acsf_tangent may carry (kind << PT_ACSF_KIND_SHIFT), kind from
  * pt_sf_table_kind() of the table's host copy: the launch then uses a kernel
  * compiled for that table class only (fewer registers, higher occupancy).
  * Results are those of PT_ACSF_GENERIC; a kernel that meets a group outside

@@ -1112,7 +1112,16 @@ int launch(KArgs a, hipStream_t stream) {
 
 template <typename R>
 int run_kind(int mode, int kind, KArgs& a, hipStream_t st) {
-    if (mode == MODE_TAN) return launch<R, MODE_TAN, PT_ACSF_GENERIC>(a, st);
+    if (mode == MODE_TAN) {
+        switch (kind) {
+        case PT_ACSF_GENERIC: return launch<R, MODE_TAN, PT_ACSF_GENERIC>(a, st);
+        case PT_ACSF_TANH_P4: return launch<R, MODE_TAN, PT_ACSF_TANH_P4>(a, st);
+        case PT_ACSF_TANH_Z1N2: return launch<R, MODE_TAN, PT_ACSF_TANH_Z1N2>(a, st);
+        case PT_ACSF_TANH_Z14: return launch<R, MODE_TAN, PT_ACSF_TANH_Z14>(a, st);
+        }
+        set_error("acsf: unknown kernel kind %d", kind);
+        return PT_ERR_ARG;
+    }
     switch (kind) {
     case PT_ACSF_GENERIC:
         return mode == MODE_FWD ? launch<R, MODE_FWD, PT_ACSF_GENERIC>(a, st) : launch<R, MODE_ADJ, PT_ACSF_GENERIC>(a, st);

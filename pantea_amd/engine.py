@@ -349,7 +349,8 @@ def acsf_tangent(nl: NeighborList, table: DeviceTable, precision: int, n_centers
         out = torch.zeros((n_centers, ld), dtype=_dtype(precision), device=nl.pos4.device)
     box_p, _keep = _box_arg(nl.box_lengths)
     _lib.check(_lib.load().pt_acsf_tangent(
-        precision, nl.pos4.data_ptr(), box_p, nl.row_off.data_ptr(), nl.nbr.data_ptr(), nl.max_row,
+        precision | (table.kind << _lib.PT_ACSF_KIND_SHIFT), nl.pos4.data_ptr(), box_p, nl.row_off.data_ptr(),
+        nl.nbr.data_ptr(), nl.max_row,
         n_centers, center_begin, _vp(centers), table.ptr, delta.data_ptr(), int(center_only),
         out.data_ptr(), ld, _lib.stream_ptr()), "pt_acsf_tangent")
     return out

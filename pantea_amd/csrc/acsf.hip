@@ -641,13 +641,6 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
     // test is the cutoff's own (it is 0 beyond rc)
     const R rc2 = X.g3 ? X.rc * X.rc * R(1.0001) : R(INFINITY);
     const int sk = X.same ? 1 : 0;
-    auto screen = [&](int ia, int ib) {
-        const V4<R> pa = S.P[ia], pb = S.P[ib];
-        R jx = pa.x - pb.x, jy = pa.y - pb.y, jz = pa.z - pb.z;
-        if (X.mic) mic3(*X.box, jx, jy, jz);
-        const R r2 = jx * jx + jy * jy + jz * jz;
-        return (r2 > R(0)) & (r2 < rc2);
-    };
     int qn = 0, base = 0;
     // one batch of up to 64 queued pairs through the body; the rest moves up
     auto drain = [&]() {
@@ -674,20 +667,38 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
     while (true) {
         if (base < npairs2) {
             const int ia = brow + q;
-            const bool in = base + lane < npairs2;
-            int dd = kDiag * d;
-            int k = q + sk + dd;
-            k = k >= ncol ? k - ncol : k;
+            const int d0 = kDiag * d;
+            // the lane's diagonals d0 + u exist for u < nu; the doubled diagonal
+            // of an even segment (the last one, half_d = h - 1) only for q < n/2
+            int nu = min(max(h - d0, 0), kDiag);
+            if (q >= half_q && half_d >= d0 && half_d < d0 + kDiag) nu = half_d - d0;
+            if (base + lane >= npairs2) nu = 0;
+            // column k = (q + sk + d0 + u) mod ncol: q + sk + d0 + u < 2 ncol for
+            // every existing diagonal, so k0 + u wraps at most once; a lane with
+            // no diagonal reads from the segment start (loads stay in the stage)
+            int k0 = q + sk + d0;
+            k0 = k0 >= ncol ? k0 - ncol : k0;
+            k0 = nu > 0 ? k0 : 0;
+            const V4<R>* pseg = S.P + bcol;
+            const unsigned eb = (unsigned)ia | ((unsigned)bcol << 16);
+#pragma unroll
             for (int u = 0; u < kDiag; ++u) {
-                const bool v = in && dd < h && !(dd == half_d && q >= half_q);
-                const bool live = v && screen(ia, bcol + k);
-                const unsigned long long b = __ballot(live);
-                if (live) S.queue[qn + lanes_below(b)] = (unsigned)ia | ((unsigned)(bcol + k) << 16);
+                // (k0 + u) mod ncol as an unsigned minimum: k0 + u - ncol is
+                // negative (huge) unless the column wraps; at most 2 past the
+                // segment start for a lane beyond nu
+                const unsigned kk = min((unsigned)(k0 + u), (unsigned)(k0 + u - ncol));
+                const V4<R> pa = S.P[ia], pb = pseg[kk];
+                R jx = pa.x - pb.x, jy = pa.y - pb.y, jz = pa.z - pb.z;
+                if (X.mic) mic3(*X.box, jx, jy, jz);
+                const R r2 = jx * jx + jy * jy + jz * jz;
+                const bool live = (u < nu) & (r2 > R(0)) & (r2 < rc2);
+                // (a ballot per compare: no lane mask rebuilt from the bool)
+                const unsigned long long b = __builtin_amdgcn_ballot_w64(u < nu) &
+                                             __builtin_amdgcn_ballot_w64(r2 > R(0)) &
+                                             __builtin_amdgcn_ballot_w64(r2 < rc2);
+                if (live) S.queue[qn + lanes_below(b)] = eb + (kk << 16);
                 qn += __popcll(b);
                 if (qn >= kWave) drain();
-                ++dd;
-                ++k;
-                k = k >= ncol ? k - ncol : k;
             }
             base += kWave;
             q += r64;

@@ -740,7 +740,7 @@ template <typename R, int MODE, int KIND>
 __device__ __forceinline__ void acsf_centre(const KArgs& a, char* smem) {
     if (sizeof(R) == 8) exp2_tab_load();
     const int lane = wave_lane();
-    const int wv = threadIdx.x >> 6;
+    const int wv = uniform(threadIdx.x >> 6);
     const int wpb = blockDim.x >> 6;
     const int c = uniform(xcd_block() * wpb + wv);
     if (c >= a.n_centers) return;

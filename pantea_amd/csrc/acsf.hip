@@ -660,8 +660,8 @@ __device__ __forceinline__ void angular_group(const Stage<R>& S, const AngGroup<
     // u = 0 .. kDiag - 1, which share the row slot and the index arithmetic;
     // the queue is drained whenever it holds a full batch, so it never holds
     // more than 63 + 64 entries.  (Measured on gfx950: 4 beats 2 and 8 in
-    // FP32; the FP64 adjoint, short of registers, does best with 2.)
-    constexpr int kDiag = (sizeof(R) == 8 && MODE == MODE_ADJ) ? 2 : 4;
+    // FP32, and 2 in the FP64 adjoint by 2.5 %.)
+    constexpr int kDiag = 4;
     const int hh = (h + kDiag - 1) / kDiag;
     const int npairs2 = nrow * hh;
     while (true) {
